@@ -1,0 +1,219 @@
+/*
+ * quickfoil_amd: MI355X (gfx950) implementation of QuickFOIL's candidate-literal evaluation
+ * path behind a C ABI.
+ *
+ * Conventions
+ *  - Every function returns an int status (qf_status).  On failure qf_last_error(ctx) holds a
+ *    message.  No C++ exception crosses this boundary.
+ *  - Pointers named d_* are device (HBM) pointers owned by the caller; h_* are host pointers.
+ *    Table columns are int32 arrays, column-major (SoA), as the reference's TableView holds them
+ *    (storage/TableView.hpp:22-88).
+ *  - All work is ordered on the context's stream.  A context is not thread-safe; calls that
+ *    return host results (counts, sizes) synchronise the stream before returning.
+ *  - Counts are uint64.  The reference keeps int32 size_type counts
+ *    (learner/CandidateLiteralInfo.hpp:21-25); qf_evaluate_literals reports QF_ERR_COUNT_RANGE
+ *    beside the (correct) 64-bit values when a count does not fit int32.
+ */
+#ifndef QUICKFOIL_AMD_H_
+#define QUICKFOIL_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QF_MAX_ARITY 6
+
+typedef enum {
+  QF_OK = 0,
+  QF_ERR_INVALID = 1,        /* bad argument */
+  QF_ERR_DEVICE = 2,         /* HIP runtime error */
+  QF_ERR_OOM = 3,            /* device allocation failed */
+  QF_ERR_OVERFLOW = 4,       /* caller-given output size does not match the result */
+  QF_ERR_COUNT_RANGE = 5,    /* a count exceeds int32 (values are still exact in uint64) */
+  QF_ERR_UNSUPPORTED = 6
+} qf_status;
+
+typedef struct qf_context qf_context;
+
+/* Opaque context: device, stream, device memory pool, cached partitions. */
+int qf_context_create(int device, qf_context** out);
+void qf_context_destroy(qf_context* ctx);
+/* Use a caller-owned hipStream_t (default priority) instead of the context's own stream. */
+int qf_context_set_stream(qf_context* ctx, void* hip_stream);
+int qf_synchronize(qf_context* ctx);
+const char* qf_last_error(const qf_context* ctx);
+/* Releases cached partitions and the pooled device memory. */
+int qf_context_trim(qf_context* ctx);
+
+/* Phase timing with HIP events recorded on the context's stream around each phase of the
+ * evaluation (relation partitioning, binding partitioning, HBM table build, probe/count).
+ * qf_context_phase_times copies up to max_phases (name, accumulated milliseconds, launches)
+ * entries and returns the number of phases recorded since the last reset. */
+int qf_context_enable_timing(qf_context* ctx, int enable);
+int qf_context_reset_timing(qf_context* ctx);
+int qf_context_phase_times(qf_context* ctx, const char** names, double* ms, int64_t* launches, int max_phases);
+
+/* A table of int32 columns in device memory (TableView, storage/TableView.hpp:22-88). */
+typedef struct {
+  const int32_t* const* columns; /* host array of num_columns device pointers */
+  int num_columns;
+  int64_t num_rows;
+} qf_table;
+
+/* ---------------------------------------------------------------------------------------
+ * Operators with the reference's layouts.
+ * ------------------------------------------------------------------------------------- */
+
+/* RadixPartition(int column_id, TableView*) (operations/RadixPartition.hpp:13-14,
+ * operations/RadixPartition.cpp:48-186).  Stable partition of d_values by
+ * Hash(v) & (2^radix_bits - 1) with the identity int32 hash (utility/Hash.hpp:20-23).
+ * d_out_pairs: 2*n int32, PartitionTuple {value, tuple_id} interleaved
+ * (storage/PartitionTuple.hpp:20-42).  d_offsets: 2^radix_bits + 1 int64 partition starts.
+ * radix_bits in [1, 16]. */
+int qf_radix_partition(qf_context* ctx, const int32_t* d_values, int64_t n, int radix_bits,
+                       int32_t* d_out_pairs, int64_t* d_offsets);
+
+/* Bucket layout of BuildHashTableOnPartitions: partition p gets P2(n_p) buckets (none when
+ * empty; storage/FoilHashTable.cpp:34-49).  Host-only: h_offsets are the partition offsets,
+ * h_bucket_offsets receives num_partitions + 1 entries.  Returns the total bucket count
+ * (or a negative qf_status). */
+int64_t qf_partition_bucket_layout(const int64_t* h_offsets, int num_partitions,
+                                   int64_t* h_bucket_offsets);
+
+/* BuildHashTableOnPartitions(int column_id, TableView*) (operations/BuildHashTable.cpp:54-93).
+ * Chained tables with the reference's exact layout: d_buckets (1-based heads, 0 = empty, at
+ * d_bucket_offsets[p] for partition p) and d_next (1-based links, partition-relative, at the
+ * partition's tuple offset).  Bit-identical to sequential head insertion in partition order. */
+int qf_build_hash_table_on_partitions(qf_context* ctx, const int32_t* d_pairs,
+                                      const int64_t* d_offsets, int radix_bits,
+                                      const int64_t* d_bucket_offsets, int64_t total_buckets,
+                                      int64_t n, int32_t* d_buckets, int32_t* d_next);
+
+/* BuildHashTableOnTable(keys, table) (operations/BuildHashTable.cpp:97-125,193-210):
+ * P2(n) buckets, bucket = Hash<num_keys>(keys, row) & (P2(n) - 1) with HashCombine
+ * (utility/Hash.hpp:25-55).  num_keys in [1, QF_MAX_ARITY]. */
+int qf_build_hash_table_on_table(qf_context* ctx, const qf_table* keys, int32_t* d_buckets,
+                                 int32_t* d_next);
+
+/* ---------------------------------------------------------------------------------------
+ * Candidate-literal evaluation.
+ * ------------------------------------------------------------------------------------- */
+
+/* One candidate literal P(v_0..v_{a-1}): a relation index and, per argument position, the
+ * clause variable (= binding-table column) or -1 for a new variable
+ * (schema/FoilLiteral.hpp:21-164, schema/FoilVariable.hpp). */
+typedef struct {
+  int relation;
+  int vars[QF_MAX_ARITY];
+} qf_literal;
+
+typedef enum {
+  QF_EVAL_BATCHED = 0,       /* whole step in one persistent launch (graph-captured) */
+  QF_EVAL_PLAIN_GROUP = 1,   /* one launch per join group (relation, join column, variable) */
+  QF_EVAL_PLAIN_LITERAL = 2  /* one launch per literal */
+} qf_eval_mode;
+
+typedef struct {
+  int mode;              /* qf_eval_mode */
+  int radix_bits;        /* 0 = autotuned; else partition bits for both sides */
+  int tile_rows;         /* 0 = autotuned; binding rows per work item */
+  int use_graph;         /* batched mode: capture and replay the step as a hipGraph */
+  int reuse_partitions;  /* 1: cache relation partitions and their tables in the context, keyed
+                            by (column pointer, rows, join column, radix bits), as the reference
+                            caches them on the fact table (learner/CandidateLiteralEvaluator.cpp:
+                            291-299).  Only for relations that stay unchanged at that address
+                            until qf_context_trim; default 0. */
+} qf_eval_options;
+
+void qf_eval_options_default(qf_eval_options* opts);
+
+/* CandidateLiteralEvaluator::Evaluate (learner/CandidateLiteralEvaluator.cpp:264-406) for all
+ * literals of a refinement step, over every clause join variable at once
+ * (learner/QuickFoil.cpp:418-440).  binding: the clause's consecutive binding table, rows
+ * < num_positive positive (schema/FoilClause.hpp:77-99).  relations[i]: background fact tables;
+ * arg_types (nullable) holds each relation's argument type ids, used for the join-key rule
+ * (schema/FoilLiteral.hpp:133-144), flattened relation-major.
+ * h_counts: num_literals x 4 uint64 in CandidateLiteralInfo field order
+ * {covered_pos, covered_neg, binding_pos, binding_neg}, literal order as given.
+ * d_semi_bits (nullable): per literal, ceil(num_rows/64) uint64 words: bit r = 1 iff binding
+ * row r has at least one matching fact (the dynamic_bitset block layout,
+ * utility/BitVectorBuilder.hpp:46-54).  The positive part is rows [0, num_positive). */
+int qf_evaluate_literals(qf_context* ctx, const qf_table* binding, int64_t num_positive,
+                         const qf_table* relations, const int* arg_types, int num_relations,
+                         const qf_literal* literals, int num_literals,
+                         const qf_eval_options* opts, uint64_t* h_counts,
+                         uint64_t* d_semi_bits);
+
+/* CountAggregator::Execute over an explicit PredicateEvaluationPlan for one join group
+ * (operations/CountAggregator.cpp:112-237, learner/PredicateEvaluationPlan.hpp:31-112).
+ * Atoms are equalities relation[atom_bg_col[k]] == binding[atom_binding_col[k]] beside the join
+ * equality relation[join_column] == binding[binding_column].  Nodes 0..num_atoms-1 are the
+ * atoms; node x >= num_atoms is the conjunction of node_left[x] and node_right[x].
+ * node_literal[x] / root_literal name an output slot (or -1).  h_counts: num_slots x 4. */
+typedef struct {
+  int relation;
+  int join_column;
+  int binding_column;
+  int num_atoms;
+  const int* atom_bg_col;
+  const int* atom_binding_col;
+  int num_nodes;
+  const int* node_left;
+  const int* node_right;
+  const int* node_literal;
+  int root_literal;
+} qf_join_group_plan;
+
+int qf_count_aggregate(qf_context* ctx, const qf_table* binding, int64_t num_positive,
+                       const qf_table* relations, int num_relations,
+                       const qf_join_group_plan* plans, int num_plans, int num_slots,
+                       const qf_eval_options* opts, uint64_t* h_counts, uint64_t* d_semi_bits);
+
+/* ---------------------------------------------------------------------------------------
+ * Extending the binding set and coverage.
+ * ------------------------------------------------------------------------------------- */
+
+/* CreateLabelAwareBindingTables (operations/MultiColumnHashJoin.cpp:117-256).  Output
+ * columns: the binding columns, then one per new variable of the literal in position order;
+ * rows: positives then negatives, in the reference's row order.  num_binding_pos/neg are the
+ * evaluator's counts and size the output exactly; a mismatch returns QF_ERR_OVERFLOW (the
+ * reference writes past its buffer there).  d_out_columns: host array of device pointers,
+ * each num_binding_pos + num_binding_neg int32. */
+int qf_create_label_aware_binding_table(qf_context* ctx, const qf_table* binding,
+                                        int64_t num_positive, const qf_table* relation,
+                                        const qf_literal* literal, int64_t num_binding_pos,
+                                        int64_t num_binding_neg, int32_t* const* d_out_columns);
+
+/* CreateBindingTable (operations/MultiColumnHashJoin.cpp:26-115): variables with id >=
+ * binding->num_columns are new columns.  Call with d_out_columns == NULL to get the row count
+ * in *h_num_rows, then again with outputs of that size. */
+int qf_create_binding_table(qf_context* ctx, const qf_table* binding, const qf_table* relation,
+                            const qf_literal* literal, int32_t* const* d_out_columns,
+                            int64_t* h_num_rows);
+
+/* LeftSemiJoin::Next over all chunks (operations/LeftSemiJoin.hpp:56-141): bit i of
+ * d_out_bits (ceil(probe rows/64) words) = probe row i has a build row with equal keys.
+ * probe_keys and build_keys hold the key columns (same count, 1..QF_MAX_ARITY). */
+int qf_left_semi_join(qf_context* ctx, const qf_table* probe_keys, const qf_table* build_keys,
+                      uint64_t* d_out_bits, int64_t* h_num_ones);
+
+/* RightSemiJoin::Next (operations/RightSemiJoin.hpp:45-82): bit j = build row j has a probe
+ * row with equal keys; d_out_bits has ceil(build rows/64) words. */
+int qf_right_semi_join(qf_context* ctx, const qf_table* probe_keys, const qf_table* build_keys,
+                       uint64_t* d_out_bits, int64_t* h_num_ones);
+
+/* BuildHashTableAfterSemiJoin (operations/BuildHashTable.cpp:127-189,212-236): among rows set
+ * in d_bits, keep the first row of each distinct key (InsertIfNotPresent order) and build the
+ * chained table over them: d_buckets (P2(n)), d_next (n; entries of rows not inserted are 0),
+ * d_first (n bytes, 1 for inserted rows). */
+int qf_build_hash_table_after_semi_join(qf_context* ctx, const qf_table* keys,
+                                        const uint64_t* d_bits, int32_t* d_buckets,
+                                        int32_t* d_next, uint8_t* d_first);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUICKFOIL_AMD_H_ */

@@ -1,0 +1,412 @@
+// Device primitives: memory pool, fills, exclusive scan, and the stable radix partition that
+// RadixPartition, the chained-table builds and the evaluator's co-partitioning all use.
+//
+// Stable partition of one tile (2048 rows, 4 waves of 64 lanes x 8 rows) per workgroup:
+//   1. histogram kernel: per-tile digit counts in LDS, written bin-major to HBM;
+//   2. exclusive scan over (bin, tile) gives each tile's output base per bin;
+//   3. scatter kernel: per-wave LDS histograms and wave-level prefix sums give every row its
+//      stable rank (peers found with one ballot per digit bit), rows are staged in LDS in
+//      bin order, then written out so consecutive lanes store consecutive addresses of a bin.
+// Digits wider than 8 bits are done as stable LSD passes of <= 8 bits.
+#include "qf_internal.h"
+
+#include <algorithm>
+
+namespace qf {
+
+// ------------------------------------------------------------------------------------------
+// Pool
+
+size_t Pool::size_class(size_t bytes) {
+  size_t c = 256;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+void* Pool::get(size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  const size_t c = size_class(bytes);
+  auto it = free_.find(c);
+  void* p = nullptr;
+  if (it != free_.end()) {
+    p = it->second;
+    free_.erase(it);
+  } else {
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      // Return cached blocks to the driver and retry once.
+      (void)hipGetLastError();
+      for (auto& kv : free_) (void)hipFree(kv.second);
+      free_.clear();
+      e = hipMalloc(&p, c);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(QF_ERR_OOM, "hipMalloc of " + std::to_string(c) + " bytes failed");
+      }
+    }
+  }
+  live_[p] = c;
+  return p;
+}
+
+void Pool::put(void* p) {
+  if (!p) return;
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_.emplace(it->second, p);
+  live_.erase(it);
+}
+
+void Pool::release_all() {
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+}
+
+DevBuf::DevBuf(Context* ctx, size_t bytes) : ctx_(ctx), bytes_(bytes) { ptr_ = ctx->pool->get(bytes); }
+DevBuf::~DevBuf() {
+  if (ptr_ && ctx_) ctx_->pool->put(ptr_);
+}
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    if (ptr_ && ctx_) ctx_->pool->put(ptr_);
+    ctx_ = o.ctx_;
+    ptr_ = o.ptr_;
+    bytes_ = o.bytes_;
+    o.ptr_ = nullptr;
+  }
+  return *this;
+}
+
+void copy_d2h(Context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  QF_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  QF_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+}
+void copy_h2d(Context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  QF_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  QF_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+}
+void sync(Context* ctx) { QF_CHECK_HIP(hipStreamSynchronize(ctx->stream)); }
+
+// ------------------------------------------------------------------------------------------
+// Fills (kernels rather than hipMemsetAsync so they can sit inside captured graphs).
+
+template <class T>
+__global__ void k_fill(T* p, T v, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <class T>
+static void fill_impl(Context* ctx, T* p, T v, int64_t n) {
+  if (n <= 0) return;
+  const int blocks = (int)std::min<int64_t>(ceil_div(n, 256), 4096);
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, ctx->stream, p, v, n);
+  QF_CHECK_HIP(hipGetLastError());
+}
+void fill_i32(Context* ctx, int32_t* p, int32_t v, int64_t n) { fill_impl(ctx, p, v, n); }
+void fill_i64(Context* ctx, int64_t* p, int64_t v, int64_t n) { fill_impl(ctx, p, v, n); }
+void fill_u64(Context* ctx, uint64_t* p, uint64_t v, int64_t n) { fill_impl(ctx, p, v, n); }
+
+// ------------------------------------------------------------------------------------------
+// Exclusive scan of int64 (three kernels, recursive over block sums).
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ inline int64_t block_exclusive_scan_i64(int64_t v, int64_t* lds, int64_t* total) {
+  // Hillis-Steele over one value per thread (256 threads).
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+    int64_t add = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  const int64_t incl = lds[t];
+  *total = lds[SCAN_THREADS - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan_reduce(const int64_t* in, int64_t n, int64_t* block_sums) {
+  __shared__ int64_t lds[SCAN_THREADS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int64_t s = 0;
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    const int64_t idx = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  int64_t total;
+  block_exclusive_scan_i64(s, lds, &total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_down(const int64_t* in, int64_t* out, int64_t n, const int64_t* block_offsets) {
+  __shared__ int64_t lds[SCAN_THREADS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t vals[SCAN_ITEMS];
+  int64_t s = 0;
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    const int64_t idx = base + i;
+    vals[i] = idx < n ? in[idx] : 0;
+    s += vals[i];
+  }
+  int64_t total;
+  int64_t run = block_exclusive_scan_i64(s, lds, &total) + (block_offsets ? block_offsets[blockIdx.x] : 0);
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    const int64_t idx = base + i;
+    if (idx < n) out[idx] = run;
+    run += vals[i];
+  }
+}
+
+void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n) {
+  if (n <= 0) return;
+  const int64_t blocks = ceil_div(n, SCAN_TILE);
+  if (blocks == 1) {
+    hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n, nullptr);
+    QF_CHECK_HIP(hipGetLastError());
+    return;
+  }
+  DevBuf sums(ctx, blocks * sizeof(int64_t));
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(SCAN_THREADS), 0, ctx->stream, in, n,
+                     sums.as<int64_t>());
+  QF_CHECK_HIP(hipGetLastError());
+  scan_exclusive_i64(ctx, sums.as<int64_t>(), sums.as<int64_t>(), blocks);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)blocks), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+                     sums.as<int64_t>());
+  QF_CHECK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------
+// Stable radix partition.
+
+constexpr int PT_THREADS = 256;
+constexpr int PT_WAVES = PT_THREADS / 64;
+constexpr int PT_IPT = 8;
+constexpr int PT_TILE = PT_THREADS * PT_IPT;
+constexpr int PT_MAXCOLS = 8;
+constexpr int PT_MAXBINS = 256;
+
+struct PassArgs {
+  const int32_t* in[PT_MAXCOLS];
+  int32_t* out[PT_MAXCOLS];
+  int ncols;      // columns written (inputs plus the generated row id, if any)
+  int tid_col;    // index of the generated row-id column, or -1
+  int key_col;
+  int out_stride;
+  int64_t n;
+  int64_t num_tiles;
+  int mixed, shift, bits;
+};
+
+__device__ inline uint32_t digit_of(int32_t k, int mixed, int shift, int bits) {
+  const uint32_t h = mixed ? mix32((uint32_t)k) : (uint32_t)k;
+  return bits == 0 ? 0u : (h >> shift) & ((1u << bits) - 1u);
+}
+
+__global__ __launch_bounds__(PT_THREADS) void k_part_hist(PassArgs a, int64_t* counts) {
+  __shared__ uint32_t hist[PT_MAXBINS];
+  const int nb = 1 << a.bits;
+  for (int b = threadIdx.x; b < nb; b += PT_THREADS) hist[b] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * PT_TILE;
+  const int32_t* key = a.in[a.key_col];
+  for (int i = 0; i < PT_IPT; ++i) {
+    const int64_t row = base + (int64_t)i * PT_THREADS + threadIdx.x;
+    if (row < a.n) atomicAdd(&hist[digit_of(key[row], a.mixed, a.shift, a.bits)], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += PT_THREADS) counts[(int64_t)b * a.num_tiles + blockIdx.x] = hist[b];
+}
+
+__global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassArgs a, const int64_t* bases) {
+  __shared__ uint32_t whist[PT_WAVES][PT_MAXBINS];
+  __shared__ uint32_t wstart[PT_WAVES][PT_MAXBINS];
+  __shared__ uint32_t lstart[PT_MAXBINS + 1];
+  __shared__ uint32_t scan_tmp[PT_MAXBINS];
+  __shared__ int32_t stage[PT_MAXCOLS][PT_TILE];
+
+  const int nb = 1 << a.bits;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  for (int b = threadIdx.x; b < nb; b += PT_THREADS)
+    for (int ww = 0; ww < PT_WAVES; ++ww) whist[ww][b] = 0;
+  __syncthreads();
+
+  const int64_t tile_base = (int64_t)blockIdx.x * PT_TILE;
+  const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
+  const int32_t* key = a.in[a.key_col];
+  uint32_t dig[PT_IPT];
+#pragma unroll
+  for (int i = 0; i < PT_IPT; ++i) {
+    const int64_t row = wave_base + (int64_t)i * 64 + lane;
+    dig[i] = row < a.n ? digit_of(key[row], a.mixed, a.shift, a.bits) : 0u;
+    if (row < a.n) atomicAdd(&whist[w][dig[i]], 1u);
+  }
+  __syncthreads();
+
+  // Tile totals per bin, exclusive scan over bins, per-wave starts.
+  uint32_t tot = 0;
+  if (threadIdx.x < nb)
+    for (int ww = 0; ww < PT_WAVES; ++ww) tot += whist[ww][threadIdx.x];
+  scan_tmp[threadIdx.x] = threadIdx.x < nb ? tot : 0;
+  __syncthreads();
+  for (int off = 1; off < PT_THREADS; off <<= 1) {
+    uint32_t add = threadIdx.x >= off ? scan_tmp[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan_tmp[threadIdx.x] += add;
+    __syncthreads();
+  }
+  if (threadIdx.x < nb) {
+    const uint32_t excl = scan_tmp[threadIdx.x] - tot;
+    lstart[threadIdx.x] = excl;
+    uint32_t run = excl;
+    for (int ww = 0; ww < PT_WAVES; ++ww) {
+      wstart[ww][threadIdx.x] = run;
+      run += whist[ww][threadIdx.x];
+    }
+  }
+  if (threadIdx.x == 0) lstart[nb] = scan_tmp[PT_THREADS - 1];
+  __syncthreads();
+
+  // Stable ranks within the wave, in row order, staged in LDS in bin order.
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int i = 0; i < PT_IPT; ++i) {
+    const int64_t row = wave_base + (int64_t)i * 64 + lane;
+    const bool valid = row < a.n;
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0) break;
+    const uint32_t d = dig[i];
+    uint64_t peers = vmask;
+    for (int k = 0; k < a.bits; ++k) {
+      const bool bit = (d >> k) & 1u;
+      const uint64_t bk = __ballot(bit);
+      peers &= bit ? bk : ~bk;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    uint32_t pos = 0;
+    if (valid) pos = wstart[w][d] + rank;
+    __builtin_amdgcn_wave_barrier();
+    if (valid) {
+      for (int c = 0; c < a.ncols; ++c)
+        stage[c][pos] = (c == a.tid_col) ? (int32_t)row : a.in[c][row];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) wstart[w][d] += (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+
+  const int64_t tile_rows = min((int64_t)PT_TILE, a.n - tile_base);
+  for (int j = threadIdx.x; j < tile_rows; j += PT_THREADS) {
+    const uint32_t d = digit_of(stage[a.key_col][j], a.mixed, a.shift, a.bits);
+    const int64_t dest = bases[(int64_t)d * a.num_tiles + blockIdx.x] + (j - (int64_t)lstart[d]);
+    for (int c = 0; c < a.ncols; ++c) a.out[c][dest * a.out_stride] = stage[c][j];
+  }
+}
+
+// offsets[b] = first row whose digit is >= b, for b in [0, num_parts]; rows are sorted by digit.
+__global__ void k_part_offsets(const int32_t* key, int stride, int64_t n, int mixed, int shift, int bits,
+                               int64_t* offsets, int64_t num_parts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0) {
+    if (i == 0)
+      for (int64_t b = 0; b <= num_parts; ++b) offsets[b] = 0;
+    return;
+  }
+  if (i >= n) return;
+  const int64_t d = digit_of(key[i * stride], mixed, shift, bits);
+  const int64_t dp = i == 0 ? -1 : (int64_t)digit_of(key[(i - 1) * stride], mixed, shift, bits);
+  for (int64_t b = dp + 1; b <= d; ++b) offsets[b] = i;
+  if (i == n - 1)
+    for (int64_t b = d + 1; b <= num_parts; ++b) offsets[b] = n;
+}
+
+static void run_pass(Context* ctx, PassArgs a) {
+  a.num_tiles = std::max<int64_t>(1, ceil_div(a.n, PT_TILE));
+  const int nb = 1 << a.bits;
+  DevBuf counts(ctx, (size_t)nb * a.num_tiles * sizeof(int64_t));
+  hipLaunchKernelGGL(k_part_hist, dim3((unsigned)a.num_tiles), dim3(PT_THREADS), 0, ctx->stream, a,
+                     counts.as<int64_t>());
+  QF_CHECK_HIP(hipGetLastError());
+  scan_exclusive_i64(ctx, counts.as<int64_t>(), counts.as<int64_t>(), (int64_t)nb * a.num_tiles);
+  hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)a.num_tiles), dim3(PT_THREADS), 0, ctx->stream, a,
+                     counts.as<int64_t>());
+  QF_CHECK_HIP(hipGetLastError());
+}
+
+static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col,
+                           bool gen_tid, int64_t num_rows, int mixed, int total_bits,
+                           int32_t* const* out_cols, int out_stride, int64_t* d_offsets) {
+  const int ncols = num_in_cols + (gen_tid ? 1 : 0);
+  QF_REQUIRE(ncols <= PT_MAXCOLS, "too many columns to partition");
+  QF_REQUIRE(total_bits >= 0 && total_bits <= 32, "bad radix bit count");
+  QF_REQUIRE(key_col >= 0 && key_col < num_in_cols, "bad key column");
+  if (num_rows > 0) {
+    const int passes = std::max(1, (total_bits + 7) / 8);
+    // Split the digit bits evenly over the passes (LSD order: low part first).
+    std::vector<int> bits(passes, total_bits / passes);
+    for (int p = 0; p < total_bits % passes; ++p) ++bits[p];
+    std::vector<DevBuf> tmp[2];
+    if (passes > 1)
+      for (int s = 0; s < 2 && s < passes - 1; ++s)
+        for (int c = 0; c < ncols; ++c) tmp[s].emplace_back(ctx, (size_t)num_rows * sizeof(int32_t));
+    int lo = 0;
+    for (int p = 0; p < passes; ++p) {
+      PassArgs a{};
+      a.n = num_rows;
+      a.key_col = key_col;
+      a.ncols = ncols;
+      a.mixed = mixed;
+      a.bits = bits[p];
+      a.shift = mixed ? (32 - total_bits + lo) : lo;
+      if (a.bits == 0) a.shift = 0;
+      if (p == 0) {
+        for (int c = 0; c < num_in_cols; ++c) a.in[c] = in_cols[c];
+        a.tid_col = gen_tid ? num_in_cols : -1;
+        if (gen_tid) a.in[num_in_cols] = nullptr;
+      } else {
+        for (int c = 0; c < ncols; ++c) a.in[c] = tmp[(p - 1) & 1][c].as<int32_t>();
+        a.tid_col = -1;
+      }
+      if (p == passes - 1) {
+        for (int c = 0; c < ncols; ++c) a.out[c] = out_cols[c];
+        a.out_stride = out_stride;
+      } else {
+        for (int c = 0; c < ncols; ++c) a.out[c] = tmp[p & 1][c].as<int32_t>();
+        a.out_stride = 1;
+      }
+      run_pass(ctx, a);
+      lo += bits[p];
+    }
+  }
+  if (d_offsets) {
+    const int64_t parts = (int64_t)1 << total_bits;
+    const int full_shift = mixed ? 32 - total_bits : 0;
+    const int64_t blocks = std::max<int64_t>(1, ceil_div(num_rows, 256));
+    hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, out_cols[key_col],
+                       out_stride, num_rows, mixed, total_bits == 0 ? 0 : full_shift, total_bits, d_offsets,
+                       parts);
+    QF_CHECK_HIP(hipGetLastError());
+  }
+}
+
+void stable_partition(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col, bool gen_tid,
+                      int64_t num_rows, int mixed, int total_bits, int32_t* const* out_cols, int out_stride,
+                      int64_t* d_offsets) {
+  partition_impl(ctx, in_cols, num_in_cols, key_col, gen_tid, num_rows, mixed, total_bits, out_cols, out_stride,
+                 d_offsets);
+}
+
+void stable_sort_by_key(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col, bool gen_tid,
+                        int64_t num_rows, int key_bits, int32_t* const* out_cols) {
+  partition_impl(ctx, in_cols, num_in_cols, key_col, gen_tid, num_rows, 0, key_bits, out_cols, 1, nullptr);
+}
+
+}  // namespace qf

@@ -1,0 +1,122 @@
+"""GPU parity of candidate-literal evaluation: counts and covered bitvectors, bit-exact."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import random_problem, uncle_problem
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["batched", "plain_group", "plain_literal"]
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).cuda()
+
+
+def _gpu_eval(ctx, binding, npos, rels, lits, **kw):
+    counts, bits = ctx.evaluate_literals([_t(c) for c in binding], npos, [[_t(c) for c in r] for r in rels], lits, **kw)
+    if bits is not None:
+        bits = bits.cpu().numpy().view(np.uint64)
+    return counts, bits
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_uncle_sample_data(qf_ctx, mode):
+    target, npos, rels, lits = uncle_problem()
+    want, want_bits = oracle.Problem(target, npos, rels, lits).bruteforce(semi=True)
+    got, bits = _gpu_eval(qf_ctx, target, npos, rels, lits, mode=mode, semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode", MODES)
+def test_random_small_vs_bruteforce(qf_ctx, seed, mode):
+    binding, npos, rels, lits = random_problem(seed, n=400, rel_shapes=((2, 300), (3, 200), (1, 40)), domain=15,
+                                               zipf=(seed % 2 == 0))
+    prob = oracle.Problem(binding, npos, rels, lits)
+    want, want_bits = prob.bruteforce(semi=True)
+    pipe, _, _ = prob.pipeline()
+    np.testing.assert_array_equal(pipe, want)
+    got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, mode=mode, semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)
+
+
+@pytest.mark.parametrize("radix_bits,tile", [(1, 64), (4, 256), (8, 1024), (12, 2048)])
+def test_radix_and_tile_choices_do_not_change_counts(qf_ctx, radix_bits, tile):
+    binding, npos, rels, lits = random_problem(21, n=20000, rel_shapes=((2, 9000), (3, 7000)), domain=400,
+                                               zipf=True, max_literals=40)
+    want, want_bits = oracle.Problem(binding, npos, rels, lits).grouped(semi=True)
+    got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, radix_bits=radix_bits, tile_rows=tile, semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)
+
+
+def test_heavy_partitions_and_skew(qf_ctx):
+    # A relation with one value holding most rows forces HBM tables for its partition.
+    rng = np.random.Generator(np.random.PCG64(5))
+    n, m = 50000, 60000
+    binding = [rng.integers(0, 50, n).astype(np.int32) for _ in range(3)]
+    hot = np.where(rng.random(m) < 0.7, 7, rng.integers(0, 50, m)).astype(np.int32)
+    rel = [hot, rng.integers(0, 50, m).astype(np.int32), rng.integers(0, 3, m).astype(np.int32)]
+    lits = [(0, [0, -1, -1]), (0, [0, 1, -1]), (0, [0, 1, 2]), (0, [-1, 2, -1]), (0, [1, 1, -1]), (0, [2, 0, 1])]
+    want, want_bits = oracle.Problem(binding, 20000, [rel], lits).grouped(semi=True)
+    for mode in MODES:
+        got, bits = _gpu_eval(qf_ctx, binding, 20000, [rel], lits, mode=mode, semi_bits=True, radix_bits=6)
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(bits, want_bits)
+
+
+def test_edge_cases(qf_ctx):
+    binding = [np.array([1, 2, 3], np.int32), np.array([4, 5, 6], np.int32)]
+    rel = [np.array([9], np.int32), np.array([9], np.int32)]
+    lits = [(0, [0, -1]), (0, [0, 1])]
+    for npos in (0, 3):  # all negative, all positive
+        want, _ = oracle.Problem(binding, npos, [rel], lits).bruteforce()
+        got, _ = _gpu_eval(qf_ctx, binding, npos, [rel], lits)
+        np.testing.assert_array_equal(got, want)
+    # Duplicate rows count separately; repeated variables; negative values.
+    binding = [np.array([-5, -5, 7, 7], np.int32), np.array([-5, 1, 7, 7], np.int32)]
+    rel = [np.array([-5, -5, 7, 7, 7], np.int32), np.array([-5, -5, 7, 1, 7], np.int32)]
+    lits = [(0, [0, 0]), (0, [1, 0]), (0, [0, -1]), (0, [-1, 1])]
+    want, want_bits = oracle.Problem(binding, 2, [rel], lits).bruteforce(semi=True)
+    got, bits = _gpu_eval(qf_ctx, binding, 2, [rel], lits, semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)
+
+
+def test_no_literals_and_empty_binding(qf_ctx):
+    binding = [np.array([1, 2], np.int32)]
+    rel = [np.array([1], np.int32)]
+    got, _ = _gpu_eval(qf_ctx, binding, 1, [rel], [])
+    assert got.shape == (0, 4)
+
+
+def test_count_aggregate_plan_matches_literals(qf_ctx):
+    binding, npos, rels, _ = random_problem(8, n=3000, num_cols=3, rel_shapes=((3, 2000),), domain=12)
+    lits_vars = [[0, -1, -1], [0, 1, -1], [0, -1, 2], [0, 1, 2], [0, 2, 1], [0, 0, 0]]
+    plan = oracle.generate_plan(3, 0, lits_vars)
+    conj = [(plan["left"][x], plan["right"][x], plan["literal"][x]) for x in range(len(plan["atoms"]), len(plan["left"]))]
+    p = {"relation": 0, "join_column": 0, "binding_column": 0, "atoms": plan["atoms"],
+         "atom_literals": plan["literal"][: len(plan["atoms"])], "conjunctions": conj,
+         "root_literal": plan["root_literal"]}
+    got, _ = qf_ctx.count_aggregate([_t(c) for c in binding], npos, [[_t(c) for c in r] for r in rels], [p],
+                                    len(lits_vars))
+    want, _ = oracle.Problem(binding, npos, rels, [(0, v) for v in lits_vars]).bruteforce()
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "zipf"])
+def test_microbench_shape_scaled(qf_ctx, dist):
+    """The microbench workload at 1/64 scale against the grouped-count closed form."""
+    from quickfoil_amd import datagen
+    w = datagen.microbench(dist, rows=262144, rel_rows=65536, num_values=16384)
+    want, want_bits = oracle.Problem(w["binding"], w["num_positive"], w["relations"], w["literals"]).grouped(semi=True)
+    for mode in MODES:
+        got, bits = _gpu_eval(qf_ctx, w["binding"], w["num_positive"], w["relations"], w["literals"], mode=mode,
+                              semi_bits=True)
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(bits, want_bits)

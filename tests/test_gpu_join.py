@@ -1,0 +1,89 @@
+"""GPU parity of binding-set extension (row for row) and the coverage semi-joins."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import random_problem
+from quickfoil_amd import QfError
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).cuda()
+
+
+@pytest.mark.parametrize("vars_", [[0, -1], [0, 1], [1, -1], [-1, 0], [0, 0], [1, 0]])
+@pytest.mark.parametrize("rel_rows", [50, 5000])
+def test_label_aware_binding_table_row_order(qf_ctx, vars_, rel_rows):
+    # rel_rows 50 < labels: hash on the relation; 5000: the relation probes both label tables.
+    binding, npos, rels, _ = random_problem(31, n=600, num_cols=2, rel_shapes=((2, rel_rows),), domain=25)
+    prob = oracle.Problem(binding, npos, rels, [(0, vars_)])
+    counts = prob.bruteforce()[0][0]
+    got_rows, want = prob.label_aware_binding_table(0, int(counts[2]), int(counts[3]))
+    out = qf_ctx.create_label_aware_binding_table([_t(c) for c in binding], npos, [_t(c) for c in rels[0]], vars_,
+                                                  int(counts[2]), int(counts[3]))
+    assert len(out) == len(want)
+    for g, w in zip(out, want):
+        np.testing.assert_array_equal(g.cpu().numpy(), w)
+
+
+def test_label_aware_binding_table_rejects_wrong_size(qf_ctx):
+    binding, npos, rels, _ = random_problem(32, n=200, num_cols=2, rel_shapes=((2, 100),), domain=10)
+    counts = oracle.Problem(binding, npos, rels, [(0, [0, -1])]).bruteforce()[0][0]
+    with pytest.raises(QfError) as e:
+        qf_ctx.create_label_aware_binding_table([_t(c) for c in binding], npos, [_t(c) for c in rels[0]], [0, -1],
+                                                int(counts[2]) - 1, int(counts[3]))
+    assert e.value.status == 4
+
+
+@pytest.mark.parametrize("vars_,rel_rows", [([0, 2], 50), ([0, 2], 900), ([1, 0], 900), ([2, 3], 300)])
+def test_create_binding_table_row_order(qf_ctx, vars_, rel_rows):
+    binding, npos, rels, _ = random_problem(33, n=400, num_cols=2, rel_shapes=((2, rel_rows),), domain=20)
+    prob = oracle.Problem(binding, npos, rels, [(0, vars_)])
+    want = prob.binding_table(0)
+    got = qf_ctx.create_binding_table([_t(c) for c in binding], [_t(c) for c in rels[0]], vars_)
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g.cpu().numpy(), w)
+
+
+@pytest.mark.parametrize("nkeys", [1, 2, 3])
+@pytest.mark.parametrize("sizes", [(1000, 300), (300, 1000), (70000, 5000)])
+def test_semi_joins(qf_ctx, nkeys, sizes):
+    rng = np.random.Generator(np.random.PCG64(nkeys * 13 + sizes[0]))
+    pk = [rng.integers(0, 40, sizes[0]).astype(np.int32) for _ in range(nkeys)]
+    bk = [rng.integers(0, 40, sizes[1]).astype(np.int32) for _ in range(nkeys)]
+    left, ones = qf_ctx.left_semi_join([_t(c) for c in pk], [_t(c) for c in bk])
+    want = oracle.left_semi_join(pk, bk)
+    np.testing.assert_array_equal(left.cpu().numpy().view(np.uint64), want)
+    assert ones == int(sum(bin(int(w)).count("1") for w in want))
+    right, ones = qf_ctx.right_semi_join([_t(c) for c in pk], [_t(c) for c in bk])
+    want = oracle.right_semi_join(pk, bk)
+    np.testing.assert_array_equal(right.cpu().numpy().view(np.uint64), want)
+
+
+def test_hash_table_after_semi_join(qf_ctx):
+    rng = np.random.Generator(np.random.PCG64(44))
+    n = 20000
+    keys = [rng.integers(0, 300, n).astype(np.int32), rng.integers(0, 4, n).astype(np.int32)]
+    bits = oracle.left_semi_join(keys, [rng.integers(0, 300, 500).astype(np.int32),
+                                        rng.integers(0, 4, 500).astype(np.int32)])
+    import torch
+    d_bits = torch.from_numpy(bits.view(np.int64).copy()).cuda()
+    buckets, nxt, first = qf_ctx.build_hash_table_after_semi_join([_t(c) for c in keys], d_bits)
+    want_first = oracle.dedup_after_semi_join(keys, bits)
+    np.testing.assert_array_equal(first.cpu().numpy(), want_first)
+    # Chains hold exactly the first occurrences of each bucket, newest first.
+    b = buckets.cpu().numpy()
+    nx = nxt.cpu().numpy()
+    nb = len(b)
+    ins = np.nonzero(want_first)[0]
+    for bucket in range(nb):
+        pos, got = int(b[bucket]) - 1, []
+        while pos >= 0:
+            got.append(pos)
+            pos = int(nx[pos]) - 1
+        want = [i for i in ins[::-1] if (oracle.hash_combine(oracle.hash_int(keys[0][i]), keys[1][i]) & (nb - 1)) == bucket]
+        assert got == want

@@ -1,0 +1,204 @@
+"""Pins the CPU oracle against the reference's known answers, brute force and closed forms."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import (brute_extended_join, chain, partitions_as_lists, radix_spec, random_problem, uncle_problem)
+
+
+def test_hash_known_answers():
+    # std::hash<int> is the identity (utility/Hash.hpp:20-23); negative values wrap to uint32.
+    assert oracle.hash_int(0) == 0
+    assert oracle.hash_int(12345) == 12345
+    assert oracle.hash_int(-1) == 0xFFFFFFFF
+    # boost hash_combine, hand-computed: seed ^ (v + 0x9e3779b9 + (seed << 6) + (seed >> 2)).
+    assert oracle.hash_combine(0, 0) == 0x9E3779B9
+    assert oracle.hash_combine(1, 2) == 0x9E3779FA
+    assert oracle.hash_combine(0xFFFFFFFF, 0) == 0xFFFFFFFF ^ ((0x9E3779B9 + 0xFFFFFFC0 + 0x3FFFFFFF) & 0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("case", radix_spec(), ids=lambda c: "%s-r%d-n%d" % (c["name"], c["radix_bits"], len(c["values"])))
+def test_radix_partition_reference_cases(case):
+    pairs, offsets = oracle.radix_partition(np.array(case["values"], dtype=np.int32), case["radix_bits"])
+    parts = partitions_as_lists(pairs, offsets)
+    for p, got in enumerate(parts):
+        want = case["partitions"].get(str(p), [])
+        # RadixPartition_test compares multisets ordered by tuple id; the partition is also stable.
+        assert got == sorted(want, key=lambda t: t[1])
+
+
+@pytest.mark.parametrize("bits", [1, 3, 7, 12])
+def test_radix_partition_large_reference_sizes(bits):
+    n = 100000
+    pairs, offsets = oracle.radix_partition(np.arange(n, dtype=np.int32), bits)
+    pairs = pairs.reshape(-1, 2)
+    parts = 1 << bits
+    for p in range(parts):
+        seg = pairs[offsets[p]:offsets[p + 1]]
+        expect = np.arange(p, n, parts)
+        assert np.array_equal(seg[:, 1], expect) and np.array_equal(seg[:, 0], expect)
+
+
+def test_radix_partition_negative_and_stability():
+    rng = np.random.Generator(np.random.PCG64(3))
+    v = rng.integers(-(2 ** 31), 2 ** 31, size=5000, dtype=np.int64).astype(np.int32)
+    pairs, offsets = oracle.radix_partition(v, 5)
+    pairs = pairs.reshape(-1, 2)
+    assert sorted(pairs[:, 1].tolist()) == list(range(5000))
+    for p in range(32):
+        seg = pairs[offsets[p]:offsets[p + 1]]
+        assert np.all((seg[:, 0].astype(np.int64) & 0xFFFFFFFF) & 31 == p)
+        assert np.all(np.diff(seg[:, 1]) > 0)
+        assert np.array_equal(v[seg[:, 1]], seg[:, 0])
+
+
+def test_partition_hash_tables_chain_invariants():
+    rng = np.random.Generator(np.random.PCG64(5))
+    v = rng.integers(0, 300, size=4000, dtype=np.int64).astype(np.int32)
+    bits = 3
+    pairs, offsets = oracle.radix_partition(v, bits)
+    buckets, boff, nxt = oracle.build_hash_table_on_partitions(pairs, offsets, bits)
+    vals = pairs.reshape(-1, 2)[:, 0]
+    for p in range(1 << bits):
+        n = offsets[p + 1] - offsets[p]
+        if n == 0:
+            assert boff[p + 1] == boff[p]
+            continue
+        nb = 1 << int(n - 1).bit_length()
+        assert boff[p + 1] - boff[p] == nb
+        b_of = ((vals[offsets[p]:offsets[p + 1]].astype(np.int64) & 0xFFFFFFFF) & ((nb - 1) << bits)) >> bits
+        for b in range(nb):
+            got = chain(buckets[boff[p]:boff[p + 1]], nxt[offsets[p]:offsets[p + 1]], b)
+            assert got == sorted(np.nonzero(b_of == b)[0].tolist(), reverse=True)
+
+
+def test_table_hash_chain_invariants():
+    rng = np.random.Generator(np.random.PCG64(6))
+    k0 = rng.integers(0, 50, size=777, dtype=np.int64).astype(np.int32)
+    k1 = rng.integers(-5, 5, size=777, dtype=np.int64).astype(np.int32)
+    buckets, nxt = oracle.build_hash_table_on_table([k0, k1])
+    nb = len(buckets)
+    assert nb == 1024
+    for b in range(nb):
+        got = chain(buckets, nxt, b)
+        want = [i for i in range(777) if (oracle.hash_combine(oracle.hash_int(k0[i]), k1[i]) & (nb - 1)) == b]
+        assert got == sorted(want, reverse=True)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("radix_bits,chunk", [(1, 7), (3, 32768), (5, 64)])
+def test_pipeline_equals_bruteforce(seed, radix_bits, chunk):
+    binding, npos, rels, lits = random_problem(seed, n=120, rel_shapes=((2, 90), (3, 60), (1, 30)), domain=9,
+                                               zipf=(seed == 3))
+    prob = oracle.Problem(binding, npos, rels, lits)
+    bf, bf_bits = prob.bruteforce(semi=True)
+    pc, pc_bits, st = prob.pipeline(radix_bits=radix_bits, chunk_size=chunk, semi=True)
+    gc, gc_bits = prob.grouped(semi=True)
+    assert st == 0
+    np.testing.assert_array_equal(pc, bf)
+    np.testing.assert_array_equal(gc, bf)
+    np.testing.assert_array_equal(pc_bits, bf_bits)
+    np.testing.assert_array_equal(gc_bits, bf_bits)
+
+
+def test_pipeline_uncle_sample_data():
+    target, npos, rels, lits = uncle_problem()
+    prob = oracle.Problem(target, npos, rels, lits)
+    bf, _ = prob.bruteforce()
+    pc, _, st = prob.pipeline()
+    assert st == 0
+    np.testing.assert_array_equal(pc, bf)
+    # Invariants: covered <= binding, covered_pos <= positives.
+    assert np.all(pc[:, 0] <= pc[:, 2]) and np.all(pc[:, 1] <= pc[:, 3])
+    assert np.all(pc[:, 0] <= npos) and np.all(pc[:, 1] <= len(target[0]) - npos)
+
+
+def test_counts_hand_example():
+    # T = [(1,2),(1,3),(2,2)] with row 0 positive; P = {(1,2),(1,2),(1,5),(2,9)}.
+    T = [np.array([1, 1, 2], np.int32), np.array([2, 3, 2], np.int32)]
+    P = [np.array([1, 1, 1, 2], np.int32), np.array([2, 2, 5, 9], np.int32)]
+    lits = [(0, [0, -1]), (0, [0, 1]), (0, [-1, 1]), (0, [1, 0])]
+    prob = oracle.Problem(T, 1, [P], lits)
+    pc, _, _ = prob.pipeline(radix_bits=2)
+    # P(X0,_): row0 -> 3 facts, row1 -> 3, row2 -> 1.  {cov_pos, cov_neg, bind_pos, bind_neg}
+    assert pc[0].tolist() == [1, 2, 3, 4]
+    # P(X0,X1): row0 (1,2) -> 2, row1 (1,3) -> 0, row2 (2,2) -> 0.
+    assert pc[1].tolist() == [1, 0, 2, 0]
+    # P(_,X1): row0 X1=2 -> 2, row1 X1=3 -> 0, row2 X1=2 -> 2.
+    assert pc[2].tolist() == [1, 1, 2, 2]
+    # P(X1,X0): row0 (2,1) -> 0, row1 (3,1) -> 0, row2 (2,2) -> 0.
+    assert pc[3].tolist() == [0, 0, 0, 0]
+
+
+def test_join_key_rule():
+    assert oracle.literal_join_key([0, 1]) == 0
+    assert oracle.literal_join_key([-1, 1]) == 1
+    assert oracle.literal_join_key([0, 1, 2], [1, 3, 3]) == 1  # largest type, first on ties
+    assert oracle.literal_join_key([-1, -1]) == -1
+
+
+def test_plan_slots_match_literal_atoms():
+    lits = [[0, -1, -1], [0, 1, -1], [0, -1, 2], [0, 1, 2], [0, 2, 1], [0, 0, 0], [0, 1, 1]]
+    plan = oracle.generate_plan(3, 0, lits)
+    atoms = plan["atoms"]
+    sets = []
+    for x in range(len(plan["left"])):
+        if plan["left"][x] < 0:
+            sets.append({x})
+        else:
+            sets.append(sets[plan["left"][x]] | sets[plan["right"][x]])
+    owners = {}
+    if plan["root_literal"] >= 0:
+        owners[plan["root_literal"]] = set()
+    for x, lit in enumerate(plan["literal"]):
+        if lit >= 0:
+            owners[lit] = {atoms[a] for a in sets[x]}
+    assert sorted(owners) == list(range(len(lits)))
+    for l, vars_ in enumerate(lits):
+        want = {(pos, v) for pos, v in enumerate(vars_) if v >= 0 and pos != 0}
+        assert owners[l] == want
+
+
+def test_plan_terminates_with_three_atoms():
+    lits = [[0, 1, 2, 3], [0, 1, -1, -1], [0, -1, 2, 3]]
+    plan = oracle.generate_plan(4, 0, lits)
+    assert sorted(x for x in plan["literal"] if x >= 0) + ([plan["root_literal"]] if plan["root_literal"] >= 0 else []) \
+        == sorted([0, 1, 2])
+
+
+@pytest.mark.parametrize("vars_", [[0, -1], [0, 1], [1, -1], [-1, 0], [0, 0]])
+@pytest.mark.parametrize("rel_rows", [40, 400])
+def test_label_aware_binding_table(vars_, rel_rows):
+    binding, npos, rels, _ = random_problem(11, n=150, num_cols=2, rel_shapes=((2, rel_rows),), domain=12)
+    prob = oracle.Problem(binding, npos, rels, [(0, vars_)])
+    counts = prob.bruteforce()[0][0]
+    got, cols = prob.label_aware_binding_table(0, int(counts[2]), int(counts[3]))
+    assert got == counts[2] + counts[3]
+    pos, neg = brute_extended_join(binding, npos, rels[0], vars_)
+    rows = list(zip(*[c.tolist() for c in cols]))
+    assert sorted(rows[: int(counts[2])]) == pos
+    assert sorted(rows[int(counts[2]):]) == neg
+    # A wrong presize is refused.
+    assert prob.label_aware_binding_table(0, int(counts[2]) + 1, int(counts[3]))[0] == -1
+
+
+def test_semi_joins_and_dedup():
+    rng = np.random.Generator(np.random.PCG64(9))
+    pk = [rng.integers(0, 15, 200).astype(np.int32), rng.integers(0, 3, 200).astype(np.int32)]
+    bk = [rng.integers(0, 15, 70).astype(np.int32), rng.integers(0, 3, 70).astype(np.int32)]
+    left = oracle.left_semi_join(pk, bk)
+    right = oracle.right_semi_join(pk, bk)
+    bset = set(zip(bk[0].tolist(), bk[1].tolist()))
+    pset = set(zip(pk[0].tolist(), pk[1].tolist()))
+    for i in range(200):
+        assert bool((int(left[i >> 6]) >> (i & 63)) & 1) == ((pk[0][i], pk[1][i]) in bset)
+    for j in range(70):
+        assert bool((int(right[j >> 6]) >> (j & 63)) & 1) == ((bk[0][j], bk[1][j]) in pset)
+    first = oracle.dedup_after_semi_join(pk, left)
+    seen = set()
+    for i in range(200):
+        is_set = bool((int(left[i >> 6]) >> (i & 63)) & 1)
+        key = (pk[0][i], pk[1][i])
+        assert first[i] == (1 if is_set and key not in seen else 0)
+        if is_set:
+            seen.add(key)
