@@ -119,7 +119,9 @@ typedef struct {
   int mode;              /* qf_eval_mode */
   int radix_bits;        /* 0 = autotuned; else partition bits for both sides */
   int tile_rows;         /* 0 = autotuned; binding rows per work item */
-  int use_graph;         /* batched mode: capture and replay the step as a hipGraph */
+  int use_graph;         /* batched mode: capture the step's device work as a hipGraph and
+                            replay it when the same call repeats (same columns, literals,
+                            options); needs reuse_partitions = 1 */
   int reuse_partitions;  /* 1: cache relation partitions and their tables in the context, keyed
                             by (column pointer, rows, join column, radix bits), as the reference
                             caches them on the fact table (learner/CandidateLiteralEvaluator.cpp:

@@ -9,20 +9,22 @@
 // integers are identical and the work no longer grows with the number of matching pairs
 // (skewed keys make that number quadratic).
 //
-// Layout.  The relation is co-partitioned with the binding table: both are stably partitioned
-// by the top R bits of mix32(join value), the relation on its join column with all columns
-// carried (SoA), the binding table on the clause join variable with the referenced columns and
-// the row id carried.  A work item is (join group, partition q, tile of binding rows of q);
-// items are counted per (group, partition) on the device, scanned, and found by binary search,
-// so no host loop touches partitions.  An item stages its binding tile in LDS, then for each
-// key-column subset the group's literals use it builds an open-addressing count table of the
-// partition's relation rows in LDS and probes it for every literal of that subset.  Partitions
-// too large for an LDS table ("heavy": where skewed keys land) have their tables built once in
-// HBM per (relation, join column, subset) by a multi-block kernel and cached with the
-// partitions; their items probe those.
+// Layout.  Relations and the binding table are co-partitioned by the top bits of mix32(join
+// value): the binding table once per clause join variable with R_T bits, carrying the referenced
+// columns and the row id; each relation on each join column with its own R_u <= R_T bits
+// (chosen from its size), carrying all its columns.  Top bits nest, so relation partition q
+// covers the contiguous binding partitions [q << (R_T - R_u), (q + 1) << (R_T - R_u)).
 //
-// Counts: per-lane u64 accumulators, 64-lane shuffle reductions, one atomic per wave and
-// literal.  Covered bitvectors (optional) are set with atomicOr at the binding row id.
+// Work.  A unit is (relation, join column) with the literal groups of every join variable that
+// uses it (one literal for the plain form).  A work item is (unit, relation partition q, chunk of
+// <= tile binding rows of q across the unit's join variables).  Item counts per (unit, q) are
+// computed and scanned on the device and items are found by binary search.  An item stages its
+// binding rows in LDS; for each key subset it builds an open-addressing count table of the
+// partition's facts in LDS (or uses the HBM table of a heavy partition, built once per
+// (relation, join column, subset) and cached), then the four waves split the (literal, 64-row
+// block) entries of that subset in literal-major order.  Per block: ballots give the covered
+// counts and, when no lane has cnt > 1, the binding counts; otherwise a shuffle reduction sums
+// cnt.  Counts live in wave-uniform registers and are flushed with one atomic per literal run.
 #include "qf_internal.h"
 
 #include <algorithm>
@@ -32,10 +34,13 @@
 namespace qf {
 
 constexpr int EV_THREADS = 256;
+constexpr int EV_WAVES = EV_THREADS / 64;
 constexpr int EV_MAXK = QF_MAX_ARITY;  // key columns of one subset tuple (distinct relation columns)
 constexpr int EV_MAXEQ = 4;            // binding-side equalities of one literal (repeated columns)
-constexpr int EV_MAXSUB = 32;          // subsets per group
-constexpr int EV_MAXTC = 12;           // binding columns staged per group (plus the row id)
+constexpr int EV_MAXSUB = 32;          // key subsets per unit
+constexpr int EV_MAXTC = 12;           // staged binding columns (plus the row id)
+constexpr int EV_MAXG = EV_MAXTC;      // join variables per unit
+constexpr int EV_MAXLU = 512;          // literals per unit
 constexpr int EV_TBL_WORDS = 12288;    // LDS words for one count table (48 KB)
 constexpr int EV_MIN_CAP = 16;
 
@@ -51,20 +56,26 @@ struct DevLit {
   int subset;
   int nk;
   int neq;
-  int tcol[EV_MAXK];        // staged binding column per key position
-  int eq[EV_MAXEQ][2];      // staged binding columns that must be equal in the row
+  int tcol[EV_MAXK];    // staged binding column per key position
+  int eq[EV_MAXEQ][2];  // staged binding columns that must be equal in the row
 };
 
-struct DevGroup {
+struct DevUGroup {
+  const int32_t* tcols[EV_MAXTC + 1];  // binding table partitioned on this join variable; [ntc] = row id
+  const int64_t* toff;
+  int lit_begin, nlit;
+};
+
+struct DevUnit {
   const int32_t* pcols[QF_MAX_ARITY];  // relation, partitioned on the join column
   const int64_t* poff;
-  const int32_t* hq;                   // per partition: heavy ordinal or -1
-  const int32_t* tcols[EV_MAXTC + 1];  // binding table partitioned on the join variable; [ntc] = row id
-  const int64_t* toff;
-  int ntc;
+  const int32_t* hq;  // per partition: heavy ordinal or -1
+  int R;              // relation radix bits
+  int shift;          // binding digits are `shift` bits finer
+  int ngroups;
   int nsub;
+  DevUGroup g[EV_MAXG];
   DevSubset sub[EV_MAXSUB];
-  int lit_begin, nlit;
 };
 
 // Table layout (cap slots): state[cap] | cnt[cap] | keys[nk][cap].  state = 1 + partition-
@@ -111,10 +122,6 @@ __device__ inline uint32_t table_probe(const uint32_t* cnt, const int32_t* keys,
 }
 
 __device__ inline uint64_t wave_sum_u64(uint64_t v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ inline uint32_t wave_sum_u32(uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
@@ -180,29 +187,47 @@ __global__ __launch_bounds__(EV_THREADS) void k_heavy_finalize(DevRel rel, DevSu
 }
 
 // ------------------------------------------------------------------------------------------
-// Work items: chunks of binding rows per (group, partition), counted and scanned on the device.
+// Work items.
 
-__global__ void k_count_items(const DevGroup* groups, int num_groups, int R, int tile, int64_t* counts) {
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t parts = (int64_t)1 << R;
-  if (id >= (int64_t)num_groups * parts) return;
-  const DevGroup& G = groups[id >> R];
-  const int64_t q = id & (parts - 1);
-  const int64_t np = G.poff[q + 1] - G.poff[q];
-  const int64_t nt = G.toff[q + 1] - G.toff[q];
-  counts[id] = (np > 0 && nt > 0) ? (nt + tile - 1) / tile : 0;
+__device__ inline int find_unit(const int64_t* unit_cells, int num_units, int64_t cell) {
+  int lo = 0, hi = num_units - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (unit_cells[mid] <= cell) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline int64_t unit_binding_rows(const DevUnit& U, int64_t q) {
+  int64_t rows = 0;
+  for (int g = 0; g < U.ngroups; ++g) rows += U.g[g].toff[(q + 1) << U.shift] - U.g[g].toff[q << U.shift];
+  return rows;
+}
+
+__global__ void k_count_items(const DevUnit* units, const int64_t* unit_cells, int num_units, int tile,
+                              int64_t* counts) {
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= unit_cells[num_units]) return;
+  const int u = find_unit(unit_cells, num_units, cell);
+  const DevUnit& U = units[u];
+  const int64_t q = cell - unit_cells[u];
+  const int64_t np = U.poff[q + 1] - U.poff[q];
+  const int64_t nt = np > 0 ? unit_binding_rows(U, q) : 0;
+  counts[cell] = (np > 0 && nt > 0) ? (nt + tile - 1) / tile : 0;
 }
 
 struct EvalArgs {
-  const DevGroup* groups;
+  const DevUnit* units;
+  const int64_t* unit_cells;  // first cell of each unit, num_units + 1 entries
+  int num_units;
   const DevLit* lits;
-  const int64_t* starts;        // exclusive scan of per-(group, partition) item counts
+  const int64_t* starts;  // exclusive scan of per-cell item counts
   const int64_t* counts;
-  int64_t num_cells;            // groups x 2^R
-  int R;
+  int ntc;                // staged binding columns (the row id follows)
   int64_t num_positive;
   unsigned long long* counts_out;  // num_slots x 4
-  unsigned long long* bits;     // num_slots x bit_words, or null
+  unsigned long long* bits;        // num_slots x bit_words, or null
   int64_t bit_words;
   int tile_rows;
 };
@@ -211,12 +236,19 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
   extern __shared__ uint32_t lds[];
   uint32_t* tbl = lds;                                                // EV_TBL_WORDS
   int32_t* tstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS);  // (ntc + 1) x tile_rows
+  __shared__ int seg_group[EV_MAXG], seg_off[EV_MAXG + 1], seg_rows[EV_MAXG], nseg_s;
+  __shared__ int64_t seg_src[EV_MAXG];
+  __shared__ int ent_start[EV_MAXG + 1];
+  __shared__ int slits[EV_MAXLU], slit_start[EV_MAXG + 1];
+
   const int lane = threadIdx.x & 63;
-  const int64_t total = a.starts[a.num_cells - 1] + a.counts[a.num_cells - 1];
+  const int wave = threadIdx.x >> 6;
+  const int64_t num_cells = a.unit_cells[a.num_units];
+  const int64_t total = a.starts[num_cells - 1] + a.counts[num_cells - 1];
+  const int ntc = a.ntc;
 
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
-    // Cell = last (group, partition) whose first item is <= it.
-    int64_t lo = 0, hi = a.num_cells - 1;
+    int64_t lo = 0, hi = num_cells - 1;
     while (lo < hi) {
       const int64_t mid = (lo + hi + 1) >> 1;
       if (a.starts[mid] <= it) lo = mid;
@@ -224,24 +256,70 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
     }
     const int64_t cell = lo;
     const int64_t chunk = it - a.starts[cell];
-    const DevGroup& G = a.groups[cell >> a.R];
-    const int64_t q = cell & (((int64_t)1 << a.R) - 1);
-    const int64_t p0 = G.poff[q];
-    const int64_t np = G.poff[q + 1] - p0;
-    const int64_t t0 = G.toff[q] + chunk * a.tile_rows;
-    const int tn = (int)min((int64_t)a.tile_rows, G.toff[q + 1] - t0);
-    const int ntc = G.ntc;
-    const int hq = G.hq[q];
+    const int u = find_unit(a.unit_cells, a.num_units, cell);
+    const DevUnit& U = a.units[u];
+    const int64_t q = cell - a.unit_cells[u];
+    const int64_t p0 = U.poff[q];
+    const int64_t np = U.poff[q + 1] - p0;
+    const int hq = U.hq[q];
 
-    __syncthreads();  // the previous item is done with the tile and the table
-    for (int c = 0; c <= ntc; ++c) {
-      const int32_t* src = G.tcols[c] + t0;
-      int32_t* dst = tstage + c * a.tile_rows;
-      for (int i = threadIdx.x; i < tn; i += EV_THREADS) dst[i] = src[i];
+    __syncthreads();  // the previous item is done with LDS
+    // Segments: the chunk's rows over the unit's join variables, in group order.
+    if (threadIdx.x == 0) {
+      const int64_t c0 = chunk * a.tile_rows, c1 = c0 + a.tile_rows;
+      int64_t run = 0;
+      int ns = 0, off = 0;
+      for (int g = 0; g < U.ngroups; ++g) {
+        const int64_t t0 = U.g[g].toff[q << U.shift], t1 = U.g[g].toff[(q + 1) << U.shift];
+        const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
+        if (hi_ > lo_) {
+          seg_group[ns] = g;
+          seg_off[ns] = off;
+          seg_rows[ns] = (int)(hi_ - lo_);
+          seg_src[ns] = t0 + (lo_ - run);
+          off += (int)(hi_ - lo_);
+          ++ns;
+        }
+        run += t1 - t0;
+      }
+      seg_off[ns] = off;
+      nseg_s = ns;
+    }
+    __syncthreads();
+    const int nseg = nseg_s;
+    // Stage the binding rows of every segment (same staged columns in every group).
+    for (int s = 0; s < nseg; ++s) {
+      const DevUGroup& G = U.g[seg_group[s]];
+      const int64_t src0 = seg_src[s];
+      const int n = seg_rows[s], dst0 = seg_off[s];
+      for (int c = 0; c <= ntc; ++c) {
+        const int32_t* src = G.tcols[c] + src0;
+        int32_t* dst = tstage + c * a.tile_rows + dst0;
+        for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
+      }
     }
 
-    for (int s = 0; s < G.nsub; ++s) {
-      const DevSubset& S = G.sub[s];
+    for (int s = 0; s < U.nsub; ++s) {
+      const DevSubset& S = U.sub[s];
+      __syncthreads();  // staging done / previous subset's probes done with table and lists
+      // Literal list of this subset, grouped by segment; entries per segment.
+      if (threadIdx.x == 0) {
+        int n = 0, e = 0;
+        for (int sg = 0; sg < nseg; ++sg) {
+          const DevUGroup& G = U.g[seg_group[sg]];
+          slit_start[sg] = n;
+          ent_start[sg] = e;
+          int k = 0;
+          for (int l = G.lit_begin; l < G.lit_begin + G.nlit; ++l)
+            if (a.lits[l].subset == s && n < EV_MAXLU) {
+              slits[n++] = l;
+              ++k;
+            }
+          e += k * ((seg_rows[sg] + 63) >> 6);
+        }
+        slit_start[nseg] = n;
+        ent_start[nseg] = e;
+      }
       const uint32_t* cnt;
       const int32_t* keys;
       uint32_t cap;
@@ -249,13 +327,12 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
         cap = table_cap_lds(np);
         uint32_t* state = tbl;
         uint32_t* c_ = tbl + cap;
-        __syncthreads();  // previous subset's probes are done with the table
         for (uint32_t i = threadIdx.x; i < 2 * cap; i += EV_THREADS) tbl[i] = 0;
         __syncthreads();
         for (int64_t r = threadIdx.x; r < np; r += EV_THREADS) {
           int32_t vals[EV_MAXK];
-          for (int k = 0; k < S.nk; ++k) vals[k] = G.pcols[S.pcol[k]][p0 + r];
-          const uint32_t slot = table_claim(state, cap, vals, S.nk, G.pcols, S.pcol, p0, r);
+          for (int k = 0; k < S.nk; ++k) vals[k] = U.pcols[S.pcol[k]][p0 + r];
+          const uint32_t slot = table_claim(state, cap, vals, S.nk, U.pcols, S.pcol, p0, r);
           atomicAdd(&c_[slot], 1u);
         }
         __syncthreads();
@@ -263,9 +340,8 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
         for (uint32_t sl = threadIdx.x; sl < cap; sl += EV_THREADS) {
           const uint32_t st = state[sl];
           if (st)
-            for (int k = 0; k < S.nk; ++k) k_[(size_t)k * cap + sl] = G.pcols[S.pcol[k]][p0 + st - 1];
+            for (int k = 0; k < S.nk; ++k) k_[(size_t)k * cap + sl] = U.pcols[S.pcol[k]][p0 + st - 1];
         }
-        __syncthreads();
         cnt = c_;
         keys = k_;
       } else {
@@ -273,44 +349,68 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
         const uint32_t* base = S.hbuf + S.hbase[hq];
         cnt = base + cap;
         keys = reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap);
-        __syncthreads();  // tile staging complete
       }
+      __syncthreads();  // table and lists ready
 
-      for (int l = G.lit_begin; l < G.lit_begin + G.nlit; ++l) {
-        const DevLit L = a.lits[l];
-        if (L.subset != s) continue;
-        uint64_t bp = 0, bn = 0;
-        uint32_t cp = 0, cn = 0;
-        for (int i = threadIdx.x; i < tn; i += EV_THREADS) {
+      // Waves split the literal-major (literal, 64-row block) entries into contiguous ranges.
+      const int E = ent_start[nseg];
+      const int e0 = (int)((int64_t)E * wave / EV_WAVES), e1 = (int)((int64_t)E * (wave + 1) / EV_WAVES);
+      int sg = 0;
+      int cur_lit = -1;
+      uint64_t bp = 0, bn = 0, cp = 0, cn = 0;
+      for (int e = e0; e < e1; ++e) {
+        while (e >= ent_start[sg + 1]) ++sg;
+        const int nblk = (seg_rows[sg] + 63) >> 6;
+        const int local = e - ent_start[sg];
+        const int li = slits[slit_start[sg] + local / nblk];
+        const int blk = local % nblk;
+        if (li != cur_lit) {
+          if (cur_lit >= 0 && lane == 0) {
+            unsigned long long* out = a.counts_out + (int64_t)a.lits[cur_lit].slot * 4;
+            if (cp) atomicAdd(&out[0], (unsigned long long)cp);
+            if (cn) atomicAdd(&out[1], (unsigned long long)cn);
+            if (bp) atomicAdd(&out[2], (unsigned long long)bp);
+            if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+          }
+          cur_lit = li;
+          bp = bn = cp = cn = 0;
+        }
+        const DevLit& L = a.lits[li];
+        const int rloc = blk * 64 + lane;
+        const bool valid = rloc < seg_rows[sg];
+        const int row = seg_off[sg] + rloc;
+        uint32_t c = 0;
+        int32_t tid = 0;
+        if (valid) {
           bool ok = true;
-          for (int e = 0; e < L.neq; ++e)
-            ok = ok && tstage[L.eq[e][0] * a.tile_rows + i] == tstage[L.eq[e][1] * a.tile_rows + i];
-          uint32_t c = 0;
+          for (int q2 = 0; q2 < L.neq; ++q2)
+            ok = ok && tstage[L.eq[q2][0] * a.tile_rows + row] == tstage[L.eq[q2][1] * a.tile_rows + row];
           if (ok) {
             int32_t vals[EV_MAXK];
-            for (int k = 0; k < L.nk; ++k) vals[k] = tstage[L.tcol[k] * a.tile_rows + i];
+            for (int k = 0; k < L.nk; ++k) vals[k] = tstage[L.tcol[k] * a.tile_rows + row];
             c = table_probe(cnt, keys, cap, vals, L.nk);
           }
-          const int32_t row = tstage[ntc * a.tile_rows + i];
-          if (row < a.num_positive) {
-            bp += c;
-            cp += (c != 0);
-          } else {
-            bn += c;
-            cn += (c != 0);
-          }
-          if (a.bits && c) atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (row >> 6)], 1ull << (row & 63));
+          tid = tstage[ntc * a.tile_rows + row];
         }
-        bp = wave_sum_u64(bp);
-        bn = wave_sum_u64(bn);
-        const uint32_t cpn = wave_sum_u32(cp | (cn << 16));  // <= 64 lanes x 16 rows per count
-        if (lane == 0) {
-          unsigned long long* out = a.counts_out + (int64_t)L.slot * 4;
-          if (cpn & 0xffffu) atomicAdd(&out[0], (unsigned long long)(cpn & 0xffffu));
-          if (cpn >> 16) atomicAdd(&out[1], (unsigned long long)(cpn >> 16));
-          if (bp) atomicAdd(&out[2], (unsigned long long)bp);
-          if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+        const bool pos = tid < a.num_positive;
+        const uint64_t mp = __ballot(c != 0 && pos), mn = __ballot(c != 0 && !pos);
+        cp += __popcll(mp);
+        cn += __popcll(mn);
+        if (__ballot(c > 1)) {
+          bp += wave_sum_u64(pos ? (uint64_t)c : 0ull);
+          bn += wave_sum_u64(pos ? 0ull : (uint64_t)c);
+        } else {
+          bp += __popcll(mp);
+          bn += __popcll(mn);
         }
+        if (a.bits && c) atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+      }
+      if (cur_lit >= 0 && lane == 0) {
+        unsigned long long* out = a.counts_out + (int64_t)a.lits[cur_lit].slot * 4;
+        if (cp) atomicAdd(&out[0], (unsigned long long)cp);
+        if (cn) atomicAdd(&out[1], (unsigned long long)cn);
+        if (bp) atomicAdd(&out[2], (unsigned long long)bp);
+        if (bn) atomicAdd(&out[3], (unsigned long long)bn);
       }
     }
   }
@@ -321,9 +421,9 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
 
 struct LitSpec {
   int slot;
-  std::vector<int> pcols;                 // distinct relation columns of the key (sorted)
-  std::vector<int> bcols;                 // binding column per key position
-  std::vector<std::pair<int, int>> eqs;   // binding columns that must be equal (repeated columns)
+  std::vector<int> pcols;                // distinct relation columns of the key (sorted)
+  std::vector<int> bcols;                // binding column per key position
+  std::vector<std::pair<int, int>> eqs;  // binding columns that must be equal (repeated columns)
 };
 
 struct GroupSpec {
@@ -364,10 +464,10 @@ static void normalise_key(LitSpec& ls) {
   QF_REQUIRE((int)ls.eqs.size() <= EV_MAXEQ, "too many repeated columns in one literal");
 }
 
-static int auto_radix_bits(int64_t max_rows) {
-  // Partitions of about 512 relation rows keep most count tables inside one LDS table.
+static int auto_radix_bits(int64_t rows) {
+  // About 512 facts per partition: most count tables fit one LDS table.
   int r = 0;
-  while (r < 16 && (max_rows >> r) > 512) ++r;
+  while (r < 16 && (rows >> r) > 512) ++r;
   return r;
 }
 
@@ -380,8 +480,8 @@ struct HeavyTables {
 // A relation partitioned on one join column, with its heavy partitions and their tables.
 struct RelPart {
   std::shared_ptr<PartitionedTable> pt;
-  DevBuf hq;                         // int32 per partition
-  std::vector<int64_t> heavy_q;      // heavy partitions
+  DevBuf hq;                     // int32 per partition
+  std::vector<int64_t> heavy_q;  // heavy partitions
   std::map<std::vector<int>, std::shared_ptr<HeavyTables>> tables;
 };
 
@@ -390,7 +490,6 @@ struct RelCache {
 };
 
 static RelCache& rel_cache(Context* ctx) {
-  // The context's generic cache slot holds this translation unit's relation cache.
   if (!ctx->eval_cache) ctx->eval_cache = std::make_shared<RelCache>();
   return *std::static_pointer_cast<RelCache>(ctx->eval_cache);
 }
@@ -478,17 +577,11 @@ static std::shared_ptr<HeavyTables> heavy_tables(Context* ctx, RelPart& rp, cons
   return ht;
 }
 
-// Binding table partitioned on column `var`, carrying `cols` (must include var) and row ids.
-static std::shared_ptr<PartitionedTable> partition_binding(Context* ctx, const qf_table& b, int var,
-                                                           const std::vector<int>& cols, int R) {
+// Binding table partitioned on column `var`, carrying `cols` (must include var) and row ids, into
+// the caller's buffers (ncols + 1 columns of rows int32, offsets 2^R + 1 int64).
+static void partition_binding(Context* ctx, const qf_table& b, int var, const std::vector<int>& cols, int R,
+                              const std::vector<int32_t*>& out_cols, int64_t* d_offsets) {
   PhaseScope ps(ctx, "partition_binding");
-  auto pt = std::make_shared<PartitionedTable>();
-  pt->radix_bits = R;
-  pt->num_columns = (int)cols.size();
-  pt->num_rows = b.num_rows;
-  pt->has_tid = true;
-  for (size_t c = 0; c <= cols.size(); ++c) pt->columns.emplace_back(ctx, std::max<int64_t>(1, b.num_rows) * 4);
-  pt->offsets = DevBuf(ctx, (((size_t)1 << R) + 1) * 8);
   // Batches of at most 6 data columns, each with the key column, so every batch partitions the
   // same way (the partition is a deterministic, stable function of the key column).
   std::vector<int> others;
@@ -499,162 +592,343 @@ static std::shared_ptr<PartitionedTable> partition_binding(Context* ctx, const q
   const int key_index = (int)(std::find(cols.begin(), cols.end(), var) - cols.begin());
   do {
     std::vector<const int32_t*> in{b.columns[var]};
-    std::vector<int32_t*> out{pt->columns[key_index].as<int32_t>()};
+    std::vector<int32_t*> out{out_cols[key_index]};
     std::vector<int> batch;
     while (pos < others.size() && batch.size() < 6) batch.push_back(others[pos++]);
     for (int c : batch) {
       in.push_back(b.columns[c]);
-      out.push_back(pt->columns[std::find(cols.begin(), cols.end(), c) - cols.begin()].as<int32_t>());
+      out.push_back(out_cols[std::find(cols.begin(), cols.end(), c) - cols.begin()]);
     }
-    if (first) out.push_back(pt->columns[cols.size()].as<int32_t>());
+    if (first) out.push_back(out_cols[cols.size()]);
     stable_partition(ctx, in.data(), (int)in.size(), 0, first, b.num_rows, 1, R, out.data(), 1,
-                     first ? pt->offsets.as<int64_t>() : nullptr);
+                     first ? d_offsets : nullptr);
     first = false;
   } while (pos < others.size());
-  return pt;
 }
 
-struct Prepared {
-  std::vector<DevGroup> groups;
-  std::vector<DevLit> lits;
-  int max_tc = 0;
-  std::vector<std::shared_ptr<PartitionedTable>> keep_bind;
+// Everything one evaluation call needs on the device; kept alive by a cached graph.
+struct StepPlan {
+  int R_T = 0;
+  int tile = 1024;
+  std::vector<int> bcols;                  // staged binding columns
+  std::vector<int> join_vars;
+  std::vector<std::vector<DevBuf>> tparts;  // per join variable: staged columns + row id
+  std::vector<DevBuf> toffs;
   std::vector<std::shared_ptr<RelPart>> keep_rel;
   std::vector<std::shared_ptr<HeavyTables>> keep_heavy;
+  std::vector<DevUnit> units;
+  std::vector<DevLit> lits;
+  std::vector<std::pair<int, int>> launches;  // unit ranges launched together
+  std::vector<int64_t> unit_cells;            // per launch-local prefix, concatenated per launch
+  DevBuf d_units, d_lits, d_unit_cells, d_cell_counts, d_cell_starts, d_counts;
+  int64_t max_cells = 0;
+  size_t lds_bytes = 0;
+  int max_blocks = 0;
+  int num_slots = 0;
+  int64_t bit_words = 0;
+  uint64_t* pinned = nullptr;
+  hipGraphExec_t graph = nullptr;
+  std::vector<DevBuf> held;  // pool blocks a captured graph uses
+  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> events;  // phase events inside the graph
+  ~StepPlan() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (pinned) (void)hipHostFree(pinned);
+    for (auto& e : events) {
+      (void)hipEventDestroy(std::get<1>(e));
+      (void)hipEventDestroy(std::get<2>(e));
+    }
+  }
 };
 
-static void prepare(Context* ctx, const qf_table& binding, const qf_table* relations,
-                    const std::vector<GroupSpec>& specs, int R, bool cache, Prepared& P) {
-  std::map<std::pair<int, int>, std::shared_ptr<RelPart>> rel_parts;
-  std::map<int, std::set<int>> var_cols;
+// Units: (relation, join column) with a group per join variable (batched), one group per
+// literal-group (plain group), or one literal each (plain literal).
+static void plan_step(Context* ctx, const qf_table& binding, const qf_table* relations,
+                      const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P) {
+  const bool cache = o.reuse_partitions != 0;
+  P.num_slots = num_slots;
+  P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
+  std::map<std::pair<int, int>, int> unit_R;
   for (const GroupSpec& g : specs) {
-    auto key = std::make_pair(g.relation, g.join_col);
-    if (!rel_parts.count(key)) rel_parts[key] = partition_relation(ctx, relations[g.relation], g.join_col, R, cache);
-    var_cols[g.join_var].insert(g.join_var);
+    const int R = o.radix_bits > 0 ? std::min(16, o.radix_bits) : auto_radix_bits(relations[g.relation].num_rows);
+    unit_R[std::make_pair(g.relation, g.join_col)] = R;
+    P.R_T = std::max(P.R_T, R);
+  }
+  std::set<int> bset, vset;
+  for (const GroupSpec& g : specs) {
+    vset.insert(g.join_var);
+    bset.insert(g.join_var);
     for (const LitSpec& l : g.lits) {
-      for (int c : l.bcols) var_cols[g.join_var].insert(c);
+      bset.insert(l.bcols.begin(), l.bcols.end());
       for (auto& e : l.eqs) {
-        var_cols[g.join_var].insert(e.first);
-        var_cols[g.join_var].insert(e.second);
+        bset.insert(e.first);
+        bset.insert(e.second);
       }
     }
   }
-  std::map<int, std::shared_ptr<PartitionedTable>> bind_parts;
-  std::map<int, std::vector<int>> var_col_list;
-  for (auto& kv : var_cols) {
-    std::vector<int> cols(kv.second.begin(), kv.second.end());
-    QF_REQUIRE((int)cols.size() <= EV_MAXTC, "too many binding columns referenced by one join variable");
-    var_col_list[kv.first] = cols;
-    bind_parts[kv.first] = partition_binding(ctx, binding, kv.first, cols, R);
+  P.bcols.assign(bset.begin(), bset.end());
+  P.join_vars.assign(vset.begin(), vset.end());
+  QF_REQUIRE((int)P.bcols.size() <= EV_MAXTC, "too many binding columns referenced by one step");
+  const int64_t rows = std::max<int64_t>(1, binding.num_rows);
+  for (size_t v = 0; v < P.join_vars.size(); ++v) {
+    P.tparts.emplace_back();
+    for (size_t c = 0; c <= P.bcols.size(); ++c) P.tparts.back().emplace_back(ctx, rows * 4);
+    P.toffs.emplace_back(ctx, (((size_t)1 << P.R_T) + 1) * 8);
   }
-  for (auto& kv : rel_parts) P.keep_rel.push_back(kv.second);
-  for (auto& kv : bind_parts) P.keep_bind.push_back(kv.second);
+  auto var_index = [&](int v) { return (int)(std::find(P.join_vars.begin(), P.join_vars.end(), v) - P.join_vars.begin()); };
+  auto tidx = [&](int c) { return (int)(std::find(P.bcols.begin(), P.bcols.end(), c) - P.bcols.begin()); };
 
+  std::map<std::pair<int, int>, std::shared_ptr<RelPart>> rel_parts;
+  for (auto& kv : unit_R)
+    rel_parts[kv.first] = partition_relation(ctx, relations[kv.first.first], kv.first.second, kv.second, cache);
+  for (auto& kv : rel_parts) P.keep_rel.push_back(kv.second);
+
+  // Unit membership.
+  struct UnitSpec {
+    std::pair<int, int> key;
+    std::vector<const GroupSpec*> groups;
+    std::vector<std::vector<const LitSpec*>> lits;
+  };
+  std::vector<UnitSpec> us;
+  std::map<std::pair<int, int>, int> uindex;
   for (const GroupSpec& g : specs) {
-    RelPart& rp = *rel_parts[std::make_pair(g.relation, g.join_col)];
-    const auto& bp = bind_parts[g.join_var];
-    const std::vector<int>& cols = var_col_list[g.join_var];
-    auto tidx = [&](int c) { return (int)(std::find(cols.begin(), cols.end(), c) - cols.begin()); };
-    DevGroup dg{};
-    for (int c = 0; c < rp.pt->num_columns; ++c) dg.pcols[c] = rp.pt->columns[c].as<int32_t>();
-    dg.poff = rp.pt->offsets.as<int64_t>();
-    dg.hq = rp.hq.as<int32_t>();
-    dg.ntc = (int)cols.size();
-    for (size_t c = 0; c <= cols.size(); ++c) dg.tcols[c] = bp->columns[c].as<int32_t>();
-    dg.toff = bp->offsets.as<int64_t>();
-    std::vector<std::vector<int>> subsets;
-    dg.lit_begin = (int)P.lits.size();
-    for (const LitSpec& l : g.lits) {
-      int s = (int)(std::find(subsets.begin(), subsets.end(), l.pcols) - subsets.begin());
-      if (s == (int)subsets.size()) subsets.push_back(l.pcols);
-      QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one join group");
-      DevLit dl{};
-      dl.slot = l.slot;
-      dl.subset = s;
-      dl.nk = (int)l.pcols.size();
-      for (int k = 0; k < dl.nk; ++k) dl.tcol[k] = tidx(l.bcols[k]);
-      dl.neq = (int)l.eqs.size();
-      for (int e = 0; e < dl.neq; ++e) {
-        dl.eq[e][0] = tidx(l.eqs[e].first);
-        dl.eq[e][1] = tidx(l.eqs[e].second);
+    const auto key = std::make_pair(g.relation, g.join_col);
+    if (o.mode == QF_EVAL_BATCHED) {
+      auto it = uindex.find(key);
+      int ui;
+      if (it == uindex.end()) {
+        ui = (int)us.size();
+        us.push_back(UnitSpec{key, {}, {}});
+        uindex[key] = ui;
+      } else {
+        ui = it->second;
       }
-      P.lits.push_back(dl);
+      UnitSpec& U = us[ui];
+      auto gi = std::find_if(U.groups.begin(), U.groups.end(), [&](const GroupSpec* x) { return x->join_var == g.join_var; });
+      if (gi == U.groups.end()) {
+        U.groups.push_back(&g);
+        U.lits.emplace_back();
+        gi = U.groups.end() - 1;
+      }
+      for (const LitSpec& l : g.lits) U.lits[gi - U.groups.begin()].push_back(&l);
+    } else if (o.mode == QF_EVAL_PLAIN_GROUP) {
+      us.push_back(UnitSpec{key, {&g}, {{}}});
+      for (const LitSpec& l : g.lits) us.back().lits[0].push_back(&l);
+    } else {
+      for (const LitSpec& l : g.lits) us.push_back(UnitSpec{key, {&g}, {{&l}}});
     }
-    dg.nlit = (int)P.lits.size() - dg.lit_begin;
-    dg.nsub = (int)subsets.size();
-    for (int s = 0; s < dg.nsub; ++s) {
-      dg.sub[s].nk = (int)subsets[s].size();
-      for (int k = 0; k < dg.sub[s].nk; ++k) dg.sub[s].pcol[k] = subsets[s][k];
+  }
+
+  for (const UnitSpec& S : us) {
+    RelPart& rp = *rel_parts[S.key];
+    DevUnit du{};
+    for (int c = 0; c < rp.pt->num_columns; ++c) du.pcols[c] = rp.pt->columns[c].as<int32_t>();
+    du.poff = rp.pt->offsets.as<int64_t>();
+    du.hq = rp.hq.as<int32_t>();
+    du.R = rp.pt->radix_bits;
+    du.shift = P.R_T - du.R;
+    du.ngroups = (int)S.groups.size();
+    QF_REQUIRE(du.ngroups <= EV_MAXG, "too many join variables in one unit");
+    std::vector<std::vector<int>> subsets;
+    int unit_lits = 0;
+    for (int gi = 0; gi < du.ngroups; ++gi) {
+      const int vi = var_index(S.groups[gi]->join_var);
+      DevUGroup& G = du.g[gi];
+      for (size_t c = 0; c <= P.bcols.size(); ++c) G.tcols[c] = P.tparts[vi][c].as<int32_t>();
+      G.toff = P.toffs[vi].as<int64_t>();
+      G.lit_begin = (int)P.lits.size();
+      for (const LitSpec* l : S.lits[gi]) {
+        int s = (int)(std::find(subsets.begin(), subsets.end(), l->pcols) - subsets.begin());
+        if (s == (int)subsets.size()) subsets.push_back(l->pcols);
+        QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one unit");
+        DevLit dl{};
+        dl.slot = l->slot;
+        dl.subset = s;
+        dl.nk = (int)l->pcols.size();
+        for (int k = 0; k < dl.nk; ++k) dl.tcol[k] = tidx(l->bcols[k]);
+        dl.neq = (int)l->eqs.size();
+        for (int e = 0; e < dl.neq; ++e) {
+          dl.eq[e][0] = tidx(l->eqs[e].first);
+          dl.eq[e][1] = tidx(l->eqs[e].second);
+        }
+        P.lits.push_back(dl);
+      }
+      G.nlit = (int)P.lits.size() - G.lit_begin;
+      unit_lits += G.nlit;
+    }
+    QF_REQUIRE(unit_lits <= EV_MAXLU, "too many literals in one unit");
+    du.nsub = (int)subsets.size();
+    for (int s = 0; s < du.nsub; ++s) {
+      du.sub[s].nk = (int)subsets[s].size();
+      for (int k = 0; k < du.sub[s].nk; ++k) du.sub[s].pcol[k] = subsets[s][k];
       if (!rp.heavy_q.empty()) {
         auto ht = heavy_tables(ctx, rp, subsets[s]);
-        dg.sub[s].hbuf = ht->buf.as<uint32_t>();
-        dg.sub[s].hbase = ht->base.as<int64_t>();
+        du.sub[s].hbuf = ht->buf.as<uint32_t>();
+        du.sub[s].hbase = ht->base.as<int64_t>();
         P.keep_heavy.push_back(ht);
       }
     }
-    P.max_tc = std::max(P.max_tc, dg.ntc);
-    P.groups.push_back(dg);
+    P.units.push_back(du);
   }
+
+  // Launch ranges: everything at once (batched) or unit by unit (plain forms).
+  const int U = (int)P.units.size();
+  if (o.mode == QF_EVAL_BATCHED) {
+    if (U > 0) P.launches.emplace_back(0, U);
+  } else {
+    for (int u = 0; u < U; ++u) P.launches.emplace_back(u, u + 1);
+  }
+  for (auto& lr : P.launches) {
+    int64_t c = 0;
+    for (int u = lr.first; u < lr.second; ++u) {
+      P.unit_cells.push_back(c);
+      c += (int64_t)1 << P.units[u].R;
+    }
+    P.unit_cells.push_back(c);
+    P.max_cells = std::max(P.max_cells, c);
+  }
+  P.d_units = DevBuf(ctx, std::max<size_t>(1, P.units.size()) * sizeof(DevUnit));
+  P.d_lits = DevBuf(ctx, std::max<size_t>(1, P.lits.size()) * sizeof(DevLit));
+  P.d_unit_cells = DevBuf(ctx, std::max<size_t>(1, P.unit_cells.size()) * 8);
+  if (!P.units.empty()) copy_h2d(ctx, P.d_units.get(), P.units.data(), P.units.size() * sizeof(DevUnit));
+  if (!P.lits.empty()) copy_h2d(ctx, P.d_lits.get(), P.lits.data(), P.lits.size() * sizeof(DevLit));
+  if (!P.unit_cells.empty()) copy_h2d(ctx, P.d_unit_cells.get(), P.unit_cells.data(), P.unit_cells.size() * 8);
+  P.d_cell_counts = DevBuf(ctx, std::max<int64_t>(1, P.max_cells) * 8);
+  P.d_cell_starts = DevBuf(ctx, std::max<int64_t>(1, P.max_cells) * 8);
+  P.d_counts = DevBuf(ctx, std::max(1, num_slots) * 4 * sizeof(uint64_t));
+  P.lds_bytes = (EV_TBL_WORDS + (size_t)(P.bcols.size() + 1) * P.tile) * 4;
+  QF_REQUIRE(P.lds_bytes <= 160 * 1024 - 4096, "work item does not fit in LDS");
+  QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)P.lds_bytes));
+  P.max_blocks = ctx->num_cus * std::max(1, (int)((160 * 1024) / (P.lds_bytes + 4096)));
+  P.bit_words = ceil_div(binding.num_rows, 64);
 }
+
+// Device work of one step, all on the context stream (capturable: kernels and device copies).
+static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_positive, StepPlan& P, uint64_t* d_bits,
+                         uint64_t* h_out) {
+  fill_u64(ctx, P.d_counts.as<uint64_t>(), 0, (int64_t)P.num_slots * 4);
+  if (d_bits && P.num_slots > 0) fill_u64(ctx, d_bits, 0, (int64_t)P.num_slots * P.bit_words);
+  if (!P.units.empty() && binding.num_rows > 0) {
+    for (size_t v = 0; v < P.join_vars.size(); ++v) {
+      std::vector<int32_t*> outs;
+      for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
+      partition_binding(ctx, binding, P.join_vars[v], P.bcols, P.R_T, outs, P.toffs[v].as<int64_t>());
+    }
+    EvalArgs a{};
+    a.lits = P.d_lits.as<DevLit>();
+    a.ntc = (int)P.bcols.size();
+    a.num_positive = num_positive;
+    a.counts_out = reinterpret_cast<unsigned long long*>(P.d_counts.get());
+    a.bits = reinterpret_cast<unsigned long long*>(d_bits);
+    a.bit_words = P.bit_words;
+    a.tile_rows = P.tile;
+    a.starts = P.d_cell_starts.as<int64_t>();
+    a.counts = P.d_cell_counts.as<int64_t>();
+    size_t uc = 0;
+    for (auto& lr : P.launches) {
+      const int nu = lr.second - lr.first;
+      const int64_t cells = P.unit_cells[uc + nu];
+      a.units = P.d_units.as<DevUnit>() + lr.first;
+      a.unit_cells = P.d_unit_cells.as<int64_t>() + uc;
+      a.num_units = nu;
+      uc += nu + 1;
+      if (cells == 0) continue;
+      {
+        PhaseScope ps(ctx, "work_items");
+        hipLaunchKernelGGL(k_count_items, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, ctx->stream, a.units,
+                           a.unit_cells, nu, P.tile, P.d_cell_counts.as<int64_t>());
+        QF_CHECK_HIP(hipGetLastError());
+        scan_exclusive_i64(ctx, P.d_cell_counts.as<int64_t>(), P.d_cell_starts.as<int64_t>(), cells);
+      }
+      const int blocks = (int)std::min<int64_t>(P.max_blocks, std::max<int64_t>(1, cells));
+      PhaseScope ps(ctx, "probe_count");
+      hipLaunchKernelGGL(k_eval, dim3(blocks), dim3(EV_THREADS), P.lds_bytes, ctx->stream, a);
+      QF_CHECK_HIP(hipGetLastError());
+    }
+  }
+  QF_CHECK_HIP(hipMemcpyAsync(h_out, P.d_counts.get(), (size_t)P.num_slots * 4 * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+}
+
+struct GraphCache {
+  std::string key;
+  std::unique_ptr<StepPlan> plan;
+};
 
 static void run_groups(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
                        std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, uint64_t* h_counts,
                        uint64_t* d_bits) {
-  int64_t max_rows = 0;
-  for (const GroupSpec& g : specs) max_rows = std::max(max_rows, relations[g.relation].num_rows);
-  const int R = o.radix_bits > 0 ? std::min(16, o.radix_bits) : auto_radix_bits(max_rows);
-  const int tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
-
-  DevBuf counts(ctx, std::max(1, num_slots) * 4 * sizeof(uint64_t));
-  fill_u64(ctx, counts.as<uint64_t>(), 0, (int64_t)num_slots * 4);
-  const int64_t words = ceil_div(binding.num_rows, 64);
-  if (d_bits && num_slots > 0) fill_u64(ctx, d_bits, 0, (int64_t)num_slots * words);
-
-  if (!specs.empty() && binding.num_rows > 0) {
-    Prepared P;
-    prepare(ctx, binding, relations, specs, R, o.reuse_partitions != 0, P);
-    DevBuf dgroups(ctx, P.groups.size() * sizeof(DevGroup)), dlits(ctx, std::max<size_t>(1, P.lits.size()) * sizeof(DevLit));
-    QF_CHECK_HIP(hipMemcpyAsync(dgroups.get(), P.groups.data(), P.groups.size() * sizeof(DevGroup),
-                                hipMemcpyHostToDevice, ctx->stream));
-    QF_CHECK_HIP(hipMemcpyAsync(dlits.get(), P.lits.data(), P.lits.size() * sizeof(DevLit), hipMemcpyHostToDevice,
-                                ctx->stream));
-    const int64_t parts = (int64_t)1 << R;
-    const size_t lds_bytes = (EV_TBL_WORDS + (size_t)(P.max_tc + 1) * tile) * 4;
-    QF_REQUIRE(lds_bytes <= 160 * 1024, "work item does not fit in LDS");
-    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eval),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-    const int max_blocks = ctx->num_cus * std::max(1, (int)((160 * 1024) / lds_bytes));
-    const bool batched = o.mode == QF_EVAL_BATCHED;
-    const int per_launch = batched ? (int)P.groups.size() : 1;
-    DevBuf cell_counts(ctx, (size_t)per_launch * parts * 8), cell_starts(ctx, (size_t)per_launch * parts * 8);
-    EvalArgs a{};
-    a.lits = dlits.as<DevLit>();
-    a.R = R;
-    a.num_positive = num_positive;
-    a.counts_out = reinterpret_cast<unsigned long long*>(counts.get());
-    a.bits = reinterpret_cast<unsigned long long*>(d_bits);
-    a.bit_words = words;
-    a.tile_rows = tile;
-    for (size_t g0 = 0; g0 < P.groups.size(); g0 += per_launch) {
-      const int ng = (int)std::min<size_t>(per_launch, P.groups.size() - g0);
-      const int64_t cells = (int64_t)ng * parts;
-      a.groups = dgroups.as<DevGroup>() + g0;
-      a.num_cells = cells;
-      a.starts = cell_starts.as<int64_t>();
-      a.counts = cell_counts.as<int64_t>();
-      PhaseScope ps(ctx, "probe_count");
-      hipLaunchKernelGGL(k_count_items, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, ctx->stream, a.groups, ng,
-                         R, tile, cell_counts.as<int64_t>());
-      QF_CHECK_HIP(hipGetLastError());
-      scan_exclusive_i64(ctx, cell_counts.as<int64_t>(), cell_starts.as<int64_t>(), cells);
-      // Enough blocks to fill the GPU; each block walks items with a grid stride.
-      const int blocks = (int)std::min<int64_t>(max_blocks, std::max<int64_t>(1, cells));
-      hipLaunchKernelGGL(k_eval, dim3(blocks), dim3(EV_THREADS), lds_bytes, ctx->stream, a);
-      QF_CHECK_HIP(hipGetLastError());
+  if (num_slots == 0) return;
+  // A captured step is replayed when the same call repeats: same binding and relation
+  // columns, literal set, options and output pointer.
+  std::string key;
+  QF_REQUIRE(!o.use_graph || o.reuse_partitions, "use_graph needs reuse_partitions (relations must stay unchanged)");
+  if (o.use_graph && o.mode == QF_EVAL_BATCHED) {
+    char buf[128];
+    snprintf(buf, sizeof(buf), "%lld/%lld/%d/%d/%d/%p|", (long long)binding.num_rows, (long long)num_positive,
+             o.radix_bits, o.tile_rows, o.reuse_partitions, (void*)d_bits);
+    key = buf;
+    for (int c = 0; c < binding.num_columns; ++c) key += std::to_string((uintptr_t)binding.columns[c]) + ",";
+    for (const GroupSpec& g : specs) {
+      key += std::to_string((uintptr_t)relations[g.relation].columns[0]) + ":" + std::to_string(g.join_col) + ":" +
+             std::to_string(g.join_var) + "[";
+      for (const LitSpec& l : g.lits) {
+        key += std::to_string(l.slot) + "/";
+        for (size_t k = 0; k < l.pcols.size(); ++k) key += std::to_string(l.pcols[k]) + "=" + std::to_string(l.bcols[k]) + ";";
+        for (auto& e : l.eqs) key += "e" + std::to_string(e.first) + "," + std::to_string(e.second) + ";";
+      }
+      key += "]";
     }
-    sync(ctx);  // the prepared buffers are released when this scope ends
+    auto* gc = static_cast<GraphCache*>(ctx->graph_cache.get());
+    if (gc && gc->key == key && gc->plan && gc->plan->graph) {
+      QF_CHECK_HIP(hipGraphLaunch(gc->plan->graph, ctx->stream));
+      sync(ctx);
+      if (ctx->timing) add_graph_phase_times(ctx, gc->plan->events);
+      std::memcpy(h_counts, gc->plan->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
+      return;
+    }
   }
-  copy_d2h(ctx, h_counts, counts.get(), (size_t)num_slots * 4 * sizeof(uint64_t));
+  std::unique_ptr<StepPlan> P(new StepPlan);
+  plan_step(ctx, binding, relations, specs, num_slots, o, *P);
+  QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->pinned), (size_t)num_slots * 4 * sizeof(uint64_t),
+                             hipHostMallocDefault));
+  if (!key.empty()) {
+    // Capture the step's device work once; replays skip all host planning.
+    // Phase events become event-record nodes of the graph; temporaries released during
+    // capture stay reserved for the graph's lifetime.
+    ctx->capture_events = &P->events;
+    ctx->capture_hold = &P->held;
+    QF_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    try {
+      enqueue_step(ctx, binding, num_positive, *P, d_bits, P->pinned);
+    } catch (...) {
+      hipGraph_t g;
+      (void)hipStreamEndCapture(ctx->stream, &g);
+      ctx->capture_events = nullptr;
+      ctx->capture_hold = nullptr;
+      throw;
+    }
+    hipGraph_t g;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    ctx->capture_events = nullptr;
+    ctx->capture_hold = nullptr;
+    QF_CHECK_HIP(ec);
+    hipError_t e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    QF_CHECK_HIP(e);
+    QF_CHECK_HIP(hipGraphLaunch(P->graph, ctx->stream));
+    sync(ctx);
+    if (ctx->timing) add_graph_phase_times(ctx, P->events);
+    std::memcpy(h_counts, P->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
+    auto gc = std::make_shared<GraphCache>();
+    gc->key = key;
+    gc->plan = std::move(P);
+    ctx->graph_cache = gc;
+    return;
+  }
+  enqueue_step(ctx, binding, num_positive, *P, d_bits, P->pinned);
+  sync(ctx);
+  std::memcpy(h_counts, P->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
 }
 
 }  // namespace qf

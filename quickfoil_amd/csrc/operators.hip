@@ -136,8 +136,21 @@ PhaseScope::~PhaseScope() {
     return;
   }
   (void)hipEventRecord(stop, ctx_->stream);
+  if (ctx_->capture_events) {
+    ctx_->capture_events->emplace_back(index_, start_, stop);
+    return;
+  }
   ctx_->phases[index_].pending.emplace_back(start_, stop);
   ctx_->phases[index_].launches++;
+}
+
+void add_graph_phase_times(Context* ctx, const std::vector<std::tuple<int, hipEvent_t, hipEvent_t>>& events) {
+  for (const auto& e : events) {
+    float ms = 0;
+    QF_CHECK_HIP(hipEventElapsedTime(&ms, std::get<1>(e), std::get<2>(e)));
+    ctx->phases[std::get<0>(e)].ms += ms;
+    ctx->phases[std::get<0>(e)].launches++;
+  }
 }
 
 void collect_phase_times(Context* ctx) {
@@ -204,6 +217,7 @@ void qf_context_destroy(qf_context* c) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->bg_cache.clear();
+  ctx->graph_cache.reset();
   ctx->eval_cache.reset();
   release_events(ctx);
   ctx->pool.reset();
@@ -233,6 +247,7 @@ int qf_context_trim(qf_context* c) {
   return guarded(c, [&](Context* ctx) {
     sync(ctx);
     ctx->bg_cache.clear();
+    ctx->graph_cache.reset();
     ctx->eval_cache.reset();
     ctx->pool->release_all();
   });

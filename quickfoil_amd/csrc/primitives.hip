@@ -64,7 +64,18 @@ void Pool::release_all() {
 
 DevBuf::DevBuf(Context* ctx, size_t bytes) : ctx_(ctx), bytes_(bytes) { ptr_ = ctx->pool->get(bytes); }
 DevBuf::~DevBuf() {
-  if (ptr_ && ctx_) ctx_->pool->put(ptr_);
+  if (!ptr_ || !ctx_) return;
+  if (ctx_->capture_hold) {
+    // Keep the block reserved for the graph being captured.
+    DevBuf keep;
+    keep.ctx_ = ctx_;
+    keep.ptr_ = ptr_;
+    keep.bytes_ = bytes_;
+    ptr_ = nullptr;
+    ctx_->capture_hold->push_back(std::move(keep));
+    return;
+  }
+  ctx_->pool->put(ptr_);
 }
 DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
   if (this != &o) {

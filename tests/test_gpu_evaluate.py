@@ -120,3 +120,20 @@ def test_microbench_shape_scaled(qf_ctx, dist):
                               semi_bits=True)
         np.testing.assert_array_equal(got, want)
         np.testing.assert_array_equal(bits, want_bits)
+
+
+def test_graph_replay_matches_eager(qf_ctx):
+    binding, npos, rels, lits = random_problem(41, n=30000, rel_shapes=((2, 20000), (3, 9000)), domain=300,
+                                               zipf=True, max_literals=60)
+    want, _ = oracle.Problem(binding, npos, rels, lits).grouped()
+    dev_b = [_t(c) for c in binding]
+    dev_r = [[_t(c) for c in r] for r in rels]
+    eager, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, lits)
+    np.testing.assert_array_equal(eager, want)
+    for i in range(3):
+        got, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, lits, use_graph=True, reuse_partitions=True)
+        np.testing.assert_array_equal(got, want)
+        # Eager work between replays.
+        again, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, lits, mode="plain_group")
+        np.testing.assert_array_equal(again, want)
+    qf_ctx.trim()
