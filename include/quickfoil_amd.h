@@ -127,9 +127,16 @@ typedef struct {
                             caches them on the fact table (learner/CandidateLiteralEvaluator.cpp:
                             291-299).  Only for relations that stay unchanged at that address
                             until qf_context_trim; default 0. */
+  int partition_rows;    /* 0 = default (512) or autotuned; target facts per relation partition,
+                            from which each relation's radix bits follow */
+  int autotune;          /* 1: on the first call for a workload shape, time a grid of
+                            (partition_rows, tile_rows) and keep the fastest in the context */
 } qf_eval_options;
 
 void qf_eval_options_default(qf_eval_options* opts);
+
+/* The configuration the autotuner chose for the most recent autotuned call (0s if none). */
+int qf_autotune_choice(qf_context* ctx, int* partition_rows, int* tile_rows, double* best_ms);
 
 /* CandidateLiteralEvaluator::Evaluate (learner/CandidateLiteralEvaluator.cpp:264-406) for all
  * literals of a refinement step, over every clause join variable at once

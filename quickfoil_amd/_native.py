@@ -55,6 +55,8 @@ class QfEvalOptions(ctypes.Structure):
         ("tile_rows", ctypes.c_int),
         ("use_graph", ctypes.c_int),
         ("reuse_partitions", ctypes.c_int),
+        ("partition_rows", ctypes.c_int),
+        ("autotune", ctypes.c_int),
     ]
 
 
@@ -93,6 +95,7 @@ SIGNATURES = {
     "qf_build_hash_table_on_partitions": (_I, [_P, _P, _P, _I, _P, _I64, _I64, _P, _P]),
     "qf_build_hash_table_on_table": (_I, [_P, ctypes.POINTER(QfTable), _P, _P]),
     "qf_eval_options_default": (None, [ctypes.POINTER(QfEvalOptions)]),
+    "qf_autotune_choice": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
     "qf_evaluate_literals": (_I, [_P, ctypes.POINTER(QfTable), _I64, ctypes.POINTER(QfTable), _P, _I,
                                   ctypes.POINTER(QfLiteral), _I, ctypes.POINTER(QfEvalOptions), _P, _P]),
     "qf_count_aggregate": (_I, [_P, ctypes.POINTER(QfTable), _I64, ctypes.POINTER(QfTable), _I,
@@ -241,18 +244,25 @@ class Context:
 
     # ---- evaluation ------------------------------------------------------------------------
     @staticmethod
-    def _options(mode, radix_bits, tile_rows, use_graph, reuse):
+    def _options(mode, radix_bits, tile_rows, use_graph, reuse, partition_rows=0, autotune=False):
         o = QfEvalOptions()
         o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
         o.radix_bits = radix_bits
         o.tile_rows = tile_rows
         o.use_graph = int(use_graph)
         o.reuse_partitions = int(reuse)
+        o.partition_rows = int(partition_rows)
+        o.autotune = int(autotune)
         return o
+
+    def autotune_choice(self):
+        pr, tr, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
+        self._check(self.lib.qf_autotune_choice(self.h, ctypes.byref(pr), ctypes.byref(tr), ctypes.byref(ms)))
+        return {"partition_rows": pr.value, "tile_rows": tr.value, "best_ms": ms.value}
 
     def evaluate_literals(self, binding_columns, num_positive, relations, literals, arg_types=None,
                           mode="batched", radix_bits=0, tile_rows=0, use_graph=False, reuse_partitions=False,
-                          semi_bits=False, check_range=False):
+                          semi_bits=False, check_range=False, partition_rows=0, autotune=False):
         """Returns (counts[L,4] uint64 ndarray, bits tensor or None).
 
         literals: list of (relation index, [var per position]) with -1 for new variables.
@@ -280,7 +290,7 @@ class Context:
         if semi_bits:
             words = (n + 63) // 64
             bits = t.empty(max(1, L * words), dtype=t.int64, device=binding_columns[0].device)
-        o = self._options(mode, radix_bits, tile_rows, use_graph, reuse_partitions)
+        o = self._options(mode, radix_bits, tile_rows, use_graph, reuse_partitions, partition_rows, autotune)
         st = self.lib.qf_evaluate_literals(self.h, ctypes.byref(b.t), num_positive, rel_arr, types_ptr, len(rels),
                                            lit_arr, L, ctypes.byref(o), counts.ctypes.data_as(ctypes.c_void_p),
                                            _ptr(bits))

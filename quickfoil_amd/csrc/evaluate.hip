@@ -464,12 +464,18 @@ static void normalise_key(LitSpec& ls) {
   QF_REQUIRE((int)ls.eqs.size() <= EV_MAXEQ, "too many repeated columns in one literal");
 }
 
-static int auto_radix_bits(int64_t rows) {
-  // About 512 facts per partition: most count tables fit one LDS table.
+static int auto_radix_bits(int64_t rows, int target) {
+  // About `target` facts per partition (default 512): most count tables fit one LDS table.
   int r = 0;
-  while (r < 16 && (rows >> r) > 512) ++r;
+  while (r < 16 && (rows >> r) > target) ++r;
   return r;
 }
+
+// Autotuner state: best (partition_rows, tile_rows) per workload shape.
+struct TuneCache {
+  std::map<std::string, std::tuple<int, int, double>> best;
+  std::tuple<int, int, double> last{0, 0, 0.0};
+};
 
 // HBM count tables of the heavy partitions of one (relation, join column) for one subset.
 struct HeavyTables {
@@ -649,7 +655,9 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
   std::map<std::pair<int, int>, int> unit_R;
   for (const GroupSpec& g : specs) {
-    const int R = o.radix_bits > 0 ? std::min(16, o.radix_bits) : auto_radix_bits(relations[g.relation].num_rows);
+    const int R = o.radix_bits > 0 ? std::min(16, o.radix_bits)
+                                   : auto_radix_bits(relations[g.relation].num_rows,
+                                                     o.partition_rows > 0 ? o.partition_rows : 512);
     unit_R[std::make_pair(g.relation, g.join_col)] = R;
     P.R_T = std::max(P.R_T, R);
   }
@@ -855,9 +863,81 @@ struct GraphCache {
   std::unique_ptr<StepPlan> plan;
 };
 
+static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
+                            std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o,
+                            uint64_t* h_counts, uint64_t* d_bits);
+
+static TuneCache& tune_cache(Context* ctx) {
+  if (!ctx->tune_cache) ctx->tune_cache = std::make_shared<TuneCache>();
+  return *std::static_pointer_cast<TuneCache>(ctx->tune_cache);
+}
+
+// Times one configuration of the step (device work only, HIP events on the context stream).
+static double time_config(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
+                          std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, uint64_t* h_counts) {
+  qf_eval_options t = o;
+  t.autotune = 0;
+  t.use_graph = 0;
+  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, nullptr);  // warm caches
+  hipEvent_t e0, e1;
+  QF_CHECK_HIP(hipEventCreate(&e0));
+  QF_CHECK_HIP(hipEventCreate(&e1));
+  QF_CHECK_HIP(hipEventRecord(e0, ctx->stream));
+  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, nullptr);
+  QF_CHECK_HIP(hipEventRecord(e1, ctx->stream));
+  QF_CHECK_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QF_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
 static void run_groups(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
                        std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, uint64_t* h_counts,
                        uint64_t* d_bits) {
+  if (num_slots == 0) return;
+  if (!o.autotune || o.radix_bits > 0 || o.tile_rows > 0 || o.partition_rows > 0 || specs.empty()) {
+    run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, o, h_counts, d_bits);
+    return;
+  }
+  // Workload shape: binding rows and columns, mode, and every group's relation size and literals.
+  std::string shape = std::to_string(binding.num_rows) + "/" + std::to_string(binding.num_columns) + "/" +
+                      std::to_string(o.mode) + "|";
+  for (const GroupSpec& g : specs)
+    shape += std::to_string(relations[g.relation].num_rows) + ":" + std::to_string(relations[g.relation].num_columns) +
+             ":" + std::to_string(g.join_col) + ":" + std::to_string(g.lits.size()) + ";";
+  TuneCache& tc = tune_cache(ctx);
+  auto it = tc.best.find(shape);
+  if (it == tc.best.end()) {
+    // Grid: facts per relation partition (radix bits) x binding rows per work item.
+    double best_ms = 1e30;
+    int best_pr = 512, best_tile = 1024;
+    for (int pr : {256, 512, 1024})
+      for (int tile : {512, 1024, 2048}) {
+        qf_eval_options t = o;
+        t.partition_rows = pr;
+        t.tile_rows = tile;
+        const double ms = time_config(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_pr = pr;
+          best_tile = tile;
+        }
+      }
+    it = tc.best.emplace(shape, std::make_tuple(best_pr, best_tile, best_ms)).first;
+  }
+  tc.last = it->second;
+  qf_eval_options t = o;
+  t.autotune = 0;
+  t.partition_rows = std::get<0>(it->second);
+  t.tile_rows = std::get<1>(it->second);
+  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, d_bits);
+}
+
+static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
+                            std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o,
+                            uint64_t* h_counts, uint64_t* d_bits) {
   if (num_slots == 0) return;
   // A captured step is replayed when the same call repeats: same binding and relation
   // columns, literal set, options and output pointer.
@@ -865,8 +945,8 @@ static void run_groups(Context* ctx, const qf_table& binding, int64_t num_positi
   QF_REQUIRE(!o.use_graph || o.reuse_partitions, "use_graph needs reuse_partitions (relations must stay unchanged)");
   if (o.use_graph && o.mode == QF_EVAL_BATCHED) {
     char buf[128];
-    snprintf(buf, sizeof(buf), "%lld/%lld/%d/%d/%d/%p|", (long long)binding.num_rows, (long long)num_positive,
-             o.radix_bits, o.tile_rows, o.reuse_partitions, (void*)d_bits);
+    snprintf(buf, sizeof(buf), "%lld/%lld/%d/%d/%d/%d/%p|", (long long)binding.num_rows, (long long)num_positive,
+             o.radix_bits, o.tile_rows, o.partition_rows, o.reuse_partitions, (void*)d_bits);
     key = buf;
     for (int c = 0; c < binding.num_columns; ++c) key += std::to_string((uintptr_t)binding.columns[c]) + ",";
     for (const GroupSpec& g : specs) {
@@ -944,6 +1024,17 @@ void qf_eval_options_default(qf_eval_options* o) {
   o->tile_rows = 0;
   o->use_graph = 0;
   o->reuse_partitions = 0;
+  o->partition_rows = 0;
+  o->autotune = 0;
+}
+
+int qf_autotune_choice(qf_context* c, int* partition_rows, int* tile_rows, double* best_ms) {
+  return guarded(c, [&](Context* ctx) {
+    const TuneCache& tc = tune_cache(ctx);
+    if (partition_rows) *partition_rows = std::get<0>(tc.last);
+    if (tile_rows) *tile_rows = std::get<1>(tc.last);
+    if (best_ms) *best_ms = std::get<2>(tc.last);
+  });
 }
 
 static void check_table(const qf_table* t, const char* what) {

@@ -109,6 +109,8 @@ struct Context {
   std::shared_ptr<void> eval_cache;
   // Captured evaluation step (graph and the buffers it uses), owned by evaluate.hip.
   std::shared_ptr<void> graph_cache;
+  // Autotuner choices per workload shape, owned by evaluate.hip.
+  std::shared_ptr<void> tune_cache;
   // While a step is being captured, released pool blocks move here instead of the pool.
   std::vector<DevBuf>* capture_hold = nullptr;
   bool timing = false;
