@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, help="MiB of a device copy to run once")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -144,16 +145,26 @@ def main():
     phases = ctx.phase_times()
 
     # Plain form: one launch per literal, same partitions.
-    ctx.reset_timing()
-    step("plain_literal")  # warm its code path
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for i in range(args.plain_steps):
-        plain_counts = step("plain_literal")
-        log("plain step %d done" % (i + 1))
-    torch.cuda.synchronize()
-    plain_s = (time.perf_counter() - t_start) / args.plain_steps
-    assert np.array_equal(plain_counts, ref_counts), "plain and batched counts differ"
+    plain_s = None
+    if args.plain_steps > 0:
+        ctx.reset_timing()
+        step("plain_literal")  # warm its code path
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for i in range(args.plain_steps):
+            plain_counts = step("plain_literal")
+            log("plain step %d done" % (i + 1))
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t_start) / args.plain_steps
+        assert np.array_equal(plain_counts, ref_counts), "plain and batched counts differ"
+
+    if args.copy_calibration:
+        # A plain device copy of known size, for scaling rocprof FETCH_SIZE / WRITE_SIZE.
+        src = torch.empty(args.copy_calibration * (1 << 20) // 4, dtype=torch.int32, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: %d MiB" % args.copy_calibration)
 
     algo_bytes, num_groups = algorithmic_probe_bytes(w)
     # Graph replays carry event-record nodes around each phase, so these are the timed steps'
@@ -186,9 +197,9 @@ def main():
         "data": "synthetic (seeded generators)",
         "config": {"workload": workload_name, "state": "warm (relation partitions cached)",
                    "literals": L, "join_groups": num_groups},
-        "plain_literals_per_s": L / plain_s,
-        "plain_ms_per_step": plain_s * 1e3,
-        "batched_over_plain": plain_s / batched_s,
+        "plain_literals_per_s": (L / plain_s) if plain_s else None,
+        "plain_ms_per_step": (plain_s * 1e3) if plain_s else None,
+        "batched_over_plain": (plain_s / batched_s) if plain_s else None,
         "cold_step_ms": cold_s * 1e3,
         "phase_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in phases.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

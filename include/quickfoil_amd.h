@@ -221,6 +221,21 @@ int qf_build_hash_table_after_semi_join(qf_context* ctx, const qf_table* keys,
                                         const uint64_t* d_bits, int32_t* d_buckets,
                                         int32_t* d_next, uint8_t* d_first);
 
+/* ---------------------------------------------------------------------------------------
+ * Data loading (host only).
+ * ------------------------------------------------------------------------------------- */
+
+/* LoadData (main/main.cpp:49-109): parses a relation file's text, one tuple per line with
+ * num_fields '|'-separated int32 fields (std::stoi rules, types/FromString.hpp:23-30); blank lines
+ * and lines starting with '#' are skipped; keep (nullable) drops skipped arguments.  Writes the
+ * kept fields column-major into h_columns (each of `capacity` rows) and the row count into
+ * *h_num_rows.  With h_columns == NULL it only counts rows.  A malformed line returns
+ * QF_ERR_INVALID with its 1-based number in *h_error_line; too small a capacity returns
+ * QF_ERR_OVERFLOW (the reference writes one row past its buffer there, main/main.cpp:73-80). */
+int qf_parse_relation_text(const char* text, int64_t length, int num_fields, const int* keep,
+                           int32_t* const* h_columns, int64_t capacity, int64_t* h_num_rows,
+                           int64_t* h_error_line);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,8 @@ SIGNATURES = {
     "qf_left_semi_join": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_right_semi_join": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_build_hash_table_after_semi_join": (_I, [_P, ctypes.POINTER(QfTable), _P, _P, _P, _P]),
+    "qf_parse_relation_text": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _I64, ctypes.POINTER(_I64),
+                                    ctypes.POINTER(_I64)]),
 }
 
 _lib = None
