@@ -959,10 +959,17 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
       }
       key += "]";
     }
+    if (!ctx->graph_stream) {
+      QF_CHECK_HIP(hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking));
+      QF_CHECK_HIP(hipEventCreateWithFlags(&ctx->graph_order_event, hipEventDisableTiming));
+    }
+    // The graph runs on the private stream after everything already queued on the caller's.
+    QF_CHECK_HIP(hipEventRecord(ctx->graph_order_event, ctx->stream));
+    QF_CHECK_HIP(hipStreamWaitEvent(ctx->graph_stream, ctx->graph_order_event, 0));
     auto* gc = static_cast<GraphCache*>(ctx->graph_cache.get());
     if (gc && gc->key == key && gc->plan && gc->plan->graph) {
-      QF_CHECK_HIP(hipGraphLaunch(gc->plan->graph, ctx->stream));
-      sync(ctx);
+      QF_CHECK_HIP(hipGraphLaunch(gc->plan->graph, ctx->graph_stream));
+      QF_CHECK_HIP(hipStreamSynchronize(ctx->graph_stream));
       if (ctx->timing) add_graph_phase_times(ctx, gc->plan->events);
       std::memcpy(h_counts, gc->plan->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
       return;
@@ -978,26 +985,33 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
     // capture stay reserved for the graph's lifetime.
     ctx->capture_events = &P->events;
     ctx->capture_hold = &P->held;
+    hipStream_t user_stream = ctx->stream;
+    // Planning may have queued work on the caller's stream: order the graph after it too.
+    QF_CHECK_HIP(hipEventRecord(ctx->graph_order_event, user_stream));
+    QF_CHECK_HIP(hipStreamWaitEvent(ctx->graph_stream, ctx->graph_order_event, 0));
+    ctx->stream = ctx->graph_stream;  // the step's work is captured from the private stream
     QF_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     try {
       enqueue_step(ctx, binding, num_positive, *P, d_bits, P->pinned);
     } catch (...) {
       hipGraph_t g;
       (void)hipStreamEndCapture(ctx->stream, &g);
+      ctx->stream = user_stream;
       ctx->capture_events = nullptr;
       ctx->capture_hold = nullptr;
       throw;
     }
     hipGraph_t g;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    ctx->stream = user_stream;
     ctx->capture_events = nullptr;
     ctx->capture_hold = nullptr;
     QF_CHECK_HIP(ec);
     hipError_t e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     QF_CHECK_HIP(e);
-    QF_CHECK_HIP(hipGraphLaunch(P->graph, ctx->stream));
-    sync(ctx);
+    QF_CHECK_HIP(hipGraphLaunch(P->graph, ctx->graph_stream));
+    QF_CHECK_HIP(hipStreamSynchronize(ctx->graph_stream));
     if (ctx->timing) add_graph_phase_times(ctx, P->events);
     std::memcpy(h_counts, P->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
     auto gc = std::make_shared<GraphCache>();

@@ -221,6 +221,11 @@ void qf_context_destroy(qf_context* c) {
   ctx->eval_cache.reset();
   release_events(ctx);
   ctx->pool.reset();
+  if (ctx->graph_stream) {
+    (void)hipStreamSynchronize(ctx->graph_stream);
+    (void)hipStreamDestroy(ctx->graph_stream);
+  }
+  if (ctx->graph_order_event) (void)hipEventDestroy(ctx->graph_order_event);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

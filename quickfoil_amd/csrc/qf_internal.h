@@ -111,6 +111,10 @@ struct Context {
   std::shared_ptr<void> graph_cache;
   // Autotuner choices per workload shape, owned by evaluate.hip.
   std::shared_ptr<void> tune_cache;
+  // Private non-blocking stream for captured graphs (the caller's stream may be the legacy
+  // default stream, which cannot be captured); ordered after the caller's stream by an event.
+  hipStream_t graph_stream = nullptr;
+  hipEvent_t graph_order_event = nullptr;
   // While a step is being captured, released pool blocks move here instead of the pool.
   std::vector<DevBuf>* capture_hold = nullptr;
   bool timing = false;
