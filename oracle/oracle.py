@@ -173,6 +173,8 @@ class Problem:
 
     def binding_table(self, literal):
         rows = lib().orc_create_binding_table(ctypes.byref(self.inp), literal, None)
+        if rows < 0:
+            raise ValueError("literal has no bound variable")
         arity = self.arity[self.lit_rel[literal]]
         vars_ = self.lit_vars[literal][:arity]
         ncols = len(self.b.arrs) + int(np.sum((vars_ < 0) | (vars_ >= len(self.b.arrs))))

@@ -696,6 +696,7 @@ int64_t orc_create_label_aware_binding_table(const orc_eval_input* in, int liter
     pos_cols.push_back(in->binding_columns[c]);
     neg_cols.push_back(in->binding_columns[c] + npos);
   }
+  if (bg_keys.empty()) return -1;  // a literal with no bound variable is never extended
   const int ncols = C + (int)unbound.size();
   // Outputs: positives at [0, num_binding_pos), negatives after (:144-159).
   std::vector<std::vector<int32_t>> pos_out(ncols), neg_out(ncols);
@@ -750,6 +751,7 @@ int64_t orc_create_binding_table(const orc_eval_input* in, int literal, int32_t*
       unbound.push_back(i);
     }
   }
+  if (lit_keys.empty()) return -1;  // a literal with no bound variable is never extended
   for (int i = 0; i < arity; ++i) bg_cols.push_back(rel_col(in, rel, i));
   for (int c = 0; c < C; ++c) bind_cols.push_back(in->binding_columns[c]);
   std::vector<int64_t> pt, bt;

@@ -38,7 +38,7 @@ def test_label_aware_binding_table_rejects_wrong_size(qf_ctx):
     assert e.value.status == 4
 
 
-@pytest.mark.parametrize("vars_,rel_rows", [([0, 2], 50), ([0, 2], 900), ([1, 0], 900), ([2, 3], 300)])
+@pytest.mark.parametrize("vars_,rel_rows", [([0, 2], 50), ([0, 2], 900), ([1, 0], 900), ([2, 0], 300)])
 def test_create_binding_table_row_order(qf_ctx, vars_, rel_rows):
     binding, npos, rels, _ = random_problem(33, n=400, num_cols=2, rel_shapes=((2, rel_rows),), domain=20)
     prob = oracle.Problem(binding, npos, rels, [(0, vars_)])
@@ -47,6 +47,13 @@ def test_create_binding_table_row_order(qf_ctx, vars_, rel_rows):
     assert len(got) == len(want)
     for g, w in zip(got, want):
         np.testing.assert_array_equal(g.cpu().numpy(), w)
+
+
+def test_literal_without_bound_variable_is_rejected(qf_ctx):
+    binding, npos, rels, _ = random_problem(34, n=50, num_cols=2, rel_shapes=((2, 40),), domain=5)
+    with pytest.raises(QfError) as e:
+        qf_ctx.create_binding_table([_t(c) for c in binding], [_t(c) for c in rels[0]], [2, 3])
+    assert e.value.status == 1
 
 
 @pytest.mark.parametrize("nkeys", [1, 2, 3])
