@@ -142,6 +142,16 @@ def main():
     torch.cuda.synchronize()
     batched_s = (time.perf_counter() - t_start) / args.steps
     assert np.array_equal(counts, ref_counts), "batched counts changed between steps"
+    # Per-kernel times: HIP events around every kernel launch of the same steps run eagerly
+    # (events cannot be timed inside a graph replay); the rocprof trace of this command
+    # cross-checks them.
+    ctx.reset_timing()
+    for i in range(args.steps):
+        eager_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], w["literals"],
+                                                mode="batched", reuse_partitions=True, radix_bits=args.radix_bits,
+                                                tile_rows=args.tile_rows)
+    torch.cuda.synchronize()
+    assert np.array_equal(eager_counts, ref_counts), "eager and graph counts differ"
     phases = ctx.phase_times()
 
     # Plain form: one launch per literal, same partitions.
@@ -167,8 +177,6 @@ def main():
         log("copy calibration: %d MiB" % args.copy_calibration)
 
     algo_bytes, num_groups = algorithmic_probe_bytes(w)
-    # Graph replays carry event-record nodes around each phase, so these are the timed steps'
-    # own kernel times.
     probe_ms, probe_launches = phases.get("probe_count", (0.0, 0))
     probe_ms_per_step = probe_ms / max(1, args.steps)
     achieved = algo_bytes / (probe_ms_per_step * 1e-3) / 1e9 if probe_ms_per_step > 0 else 0.0

@@ -635,14 +635,9 @@ struct StepPlan {
   uint64_t* pinned = nullptr;
   hipGraphExec_t graph = nullptr;
   std::vector<DevBuf> held;  // pool blocks a captured graph uses
-  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> events;  // phase events inside the graph
   ~StepPlan() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (pinned) (void)hipHostFree(pinned);
-    for (auto& e : events) {
-      (void)hipEventDestroy(std::get<1>(e));
-      (void)hipEventDestroy(std::get<2>(e));
-    }
   }
 };
 
@@ -970,7 +965,6 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
     if (gc && gc->key == key && gc->plan && gc->plan->graph) {
       QF_CHECK_HIP(hipGraphLaunch(gc->plan->graph, ctx->graph_stream));
       QF_CHECK_HIP(hipStreamSynchronize(ctx->graph_stream));
-      if (ctx->timing) add_graph_phase_times(ctx, gc->plan->events);
       std::memcpy(h_counts, gc->plan->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
       return;
     }
@@ -981,9 +975,11 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
                              hipHostMallocDefault));
   if (!key.empty()) {
     // Capture the step's device work once; replays skip all host planning.
-    // Phase events become event-record nodes of the graph; temporaries released during
-    // capture stay reserved for the graph's lifetime.
-    ctx->capture_events = &P->events;
+    // Temporaries released during capture stay reserved for the graph's lifetime.  Phase
+    // timing is off inside the graph (HIP cannot time event nodes of a graph); replays are
+    // timed as a whole by the caller.
+    const bool timing = ctx->timing;
+    ctx->timing = false;
     ctx->capture_hold = &P->held;
     hipStream_t user_stream = ctx->stream;
     // Planning may have queued work on the caller's stream: order the graph after it too.
@@ -997,14 +993,14 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
       hipGraph_t g;
       (void)hipStreamEndCapture(ctx->stream, &g);
       ctx->stream = user_stream;
-      ctx->capture_events = nullptr;
+      ctx->timing = timing;
       ctx->capture_hold = nullptr;
       throw;
     }
     hipGraph_t g;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
     ctx->stream = user_stream;
-    ctx->capture_events = nullptr;
+    ctx->timing = timing;
     ctx->capture_hold = nullptr;
     QF_CHECK_HIP(ec);
     hipError_t e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
@@ -1012,7 +1008,6 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
     QF_CHECK_HIP(e);
     QF_CHECK_HIP(hipGraphLaunch(P->graph, ctx->graph_stream));
     QF_CHECK_HIP(hipStreamSynchronize(ctx->graph_stream));
-    if (ctx->timing) add_graph_phase_times(ctx, P->events);
     std::memcpy(h_counts, P->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
     auto gc = std::make_shared<GraphCache>();
     gc->key = key;

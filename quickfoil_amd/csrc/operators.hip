@@ -136,21 +136,8 @@ PhaseScope::~PhaseScope() {
     return;
   }
   (void)hipEventRecord(stop, ctx_->stream);
-  if (ctx_->capture_events) {
-    ctx_->capture_events->emplace_back(index_, start_, stop);
-    return;
-  }
   ctx_->phases[index_].pending.emplace_back(start_, stop);
   ctx_->phases[index_].launches++;
-}
-
-void add_graph_phase_times(Context* ctx, const std::vector<std::tuple<int, hipEvent_t, hipEvent_t>>& events) {
-  for (const auto& e : events) {
-    float ms = 0;
-    QF_CHECK_HIP(hipEventElapsedTime(&ms, std::get<1>(e), std::get<2>(e)));
-    ctx->phases[std::get<0>(e)].ms += ms;
-    ctx->phases[std::get<0>(e)].launches++;
-  }
 }
 
 void collect_phase_times(Context* ctx) {

@@ -118,9 +118,6 @@ struct Context {
   // While a step is being captured, released pool blocks move here instead of the pool.
   std::vector<DevBuf>* capture_hold = nullptr;
   bool timing = false;
-  // While a step is captured with timing on, phase events are recorded into the graph and
-  // listed here (phase index, start, stop); every replay adds their elapsed times.
-  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>>* capture_events = nullptr;
   std::vector<PhaseTimer> phases;
   std::vector<hipEvent_t> event_pool;
 };
@@ -138,8 +135,6 @@ class PhaseScope {
 };
 // Resolves pending events (synchronises) into the accumulated phase times.
 void collect_phase_times(Context* ctx);
-// Adds the elapsed times of events recorded inside a replayed graph (after a synchronisation).
-void add_graph_phase_times(Context* ctx, const std::vector<std::tuple<int, hipEvent_t, hipEvent_t>>& events);
 void release_events(Context* ctx);
 
 // ------------------------------------------------------------------------------------------
