@@ -41,7 +41,8 @@ constexpr int EV_MAXSUB = 32;          // key subsets per unit
 constexpr int EV_MAXTC = 12;           // staged binding columns (plus the row id)
 constexpr int EV_MAXG = EV_MAXTC;      // join variables per unit
 constexpr int EV_MAXLU = 512;          // literals per unit
-constexpr int EV_TBL_WORDS = 12288;    // LDS words for one count table (48 KB)
+constexpr int EV_TBL_WORDS = 8192;     // LDS words for one count table (32 KB)
+constexpr int EV_PST_WORDS = 3072;     // LDS words for the partition's facts (12 KB)
 constexpr int EV_MIN_CAP = 16;
 
 struct DevSubset {
@@ -63,7 +64,7 @@ struct DevLit {
 struct DevUGroup {
   const int32_t* tcols[EV_MAXTC + 1];  // binding table partitioned on this join variable; [ntc] = row id
   const int64_t* toff;
-  int lit_begin, nlit;
+  int sbeg[EV_MAXSUB + 1];             // literals of subset s: lits[sbeg[s], sbeg[s + 1])
 };
 
 struct DevUnit {
@@ -72,6 +73,7 @@ struct DevUnit {
   const int32_t* hq;  // per partition: heavy ordinal or -1
   int R;              // relation radix bits
   int shift;          // binding digits are `shift` bits finer
+  int arity;
   int ngroups;
   int nsub;
   DevUGroup g[EV_MAXG];
@@ -189,6 +191,12 @@ __global__ __launch_bounds__(EV_THREADS) void k_heavy_finalize(DevRel rel, DevSu
 // ------------------------------------------------------------------------------------------
 // Work items.
 
+struct DevItem {
+  int unit;
+  int chunk;
+  int64_t q;
+};
+
 __device__ inline int find_unit(const int64_t* unit_cells, int num_units, int64_t cell) {
   int lo = 0, hi = num_units - 1;
   while (lo < hi) {
@@ -217,13 +225,25 @@ __global__ void k_count_items(const DevUnit* units, const int64_t* unit_cells, i
   counts[cell] = (np > 0 && nt > 0) ? (nt + tile - 1) / tile : 0;
 }
 
+// Writes the items of every cell at the cell's scanned start; the last slot gets the total.
+__global__ void k_expand_items(const int64_t* unit_cells, int num_units, const int64_t* counts,
+                               const int64_t* starts, DevItem* items, int64_t* total) {
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t cells = unit_cells[num_units];
+  if (cell >= cells) return;
+  const int64_t n = counts[cell];
+  if (cell == cells - 1) *total = starts[cell] + n;
+  if (n == 0) return;
+  const int u = find_unit(unit_cells, num_units, cell);
+  const int64_t q = cell - unit_cells[u];
+  for (int64_t c = 0; c < n; ++c) items[starts[cell] + c] = DevItem{u, (int)c, q};
+}
+
 struct EvalArgs {
   const DevUnit* units;
-  const int64_t* unit_cells;  // first cell of each unit, num_units + 1 entries
-  int num_units;
   const DevLit* lits;
-  const int64_t* starts;  // exclusive scan of per-cell item counts
-  const int64_t* counts;
+  const DevItem* items;
+  const int64_t* num_items;
   int ntc;                // staged binding columns (the row id follows)
   int64_t num_positive;
   unsigned long long* counts_out;  // num_slots x 4
@@ -232,185 +252,192 @@ struct EvalArgs {
   int tile_rows;
 };
 
-__global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
-  extern __shared__ uint32_t lds[];
-  uint32_t* tbl = lds;                                                // EV_TBL_WORDS
-  int32_t* tstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS);  // (ntc + 1) x tile_rows
-  __shared__ int seg_group[EV_MAXG], seg_off[EV_MAXG + 1], seg_rows[EV_MAXG], nseg_s;
-  __shared__ int64_t seg_src[EV_MAXG];
-  __shared__ int ent_start[EV_MAXG + 1];
-  __shared__ int slits[EV_MAXLU], slit_start[EV_MAXG + 1];
+// Per-item shared state, filled by the block's first threads.
+struct ItemShared {
+  int seg_group[EV_MAXG];
+  int seg_off[EV_MAXG + 1];
+  int seg_rows[EV_MAXG];
+  int64_t seg_src[EV_MAXG];
+  int ent_start[EV_MAXG + 1];
+  int nseg;
+};
 
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int64_t num_cells = a.unit_cells[a.num_units];
-  const int64_t total = a.starts[num_cells - 1] + a.counts[num_cells - 1];
+// Probes the current subset's table for the (literal, 64-row block) entries of this wave.
+// kLds: the table is in LDS (the pointers come from the block's LDS array); else in HBM.
+template <bool kLds>
+__device__ inline void probe_entries(const EvalArgs& a, const DevUnit& U, const ItemShared& sh, int s,
+                                     const int32_t* tstage, const uint32_t* cnt, const int32_t* keys, uint32_t cap,
+                                     int lane, int wave) {
+  const int nseg = sh.nseg;
+  const int E = sh.ent_start[nseg];
+  const int e0 = (int)((int64_t)E * wave / (EV_THREADS / 64)), e1 = (int)((int64_t)E * (wave + 1) / (EV_THREADS / 64));
   const int ntc = a.ntc;
-
-  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
-    int64_t lo = 0, hi = num_cells - 1;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) >> 1;
-      if (a.starts[mid] <= it) lo = mid;
-      else hi = mid - 1;
-    }
-    const int64_t cell = lo;
-    const int64_t chunk = it - a.starts[cell];
-    const int u = find_unit(a.unit_cells, a.num_units, cell);
-    const DevUnit& U = a.units[u];
-    const int64_t q = cell - a.unit_cells[u];
-    const int64_t p0 = U.poff[q];
-    const int64_t np = U.poff[q + 1] - p0;
-    const int hq = U.hq[q];
-
-    __syncthreads();  // the previous item is done with LDS
-    // Segments: the chunk's rows over the unit's join variables, in group order.
-    if (threadIdx.x == 0) {
-      const int64_t c0 = chunk * a.tile_rows, c1 = c0 + a.tile_rows;
-      int64_t run = 0;
-      int ns = 0, off = 0;
-      for (int g = 0; g < U.ngroups; ++g) {
-        const int64_t t0 = U.g[g].toff[q << U.shift], t1 = U.g[g].toff[(q + 1) << U.shift];
-        const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
-        if (hi_ > lo_) {
-          seg_group[ns] = g;
-          seg_off[ns] = off;
-          seg_rows[ns] = (int)(hi_ - lo_);
-          seg_src[ns] = t0 + (lo_ - run);
-          off += (int)(hi_ - lo_);
-          ++ns;
-        }
-        run += t1 - t0;
-      }
-      seg_off[ns] = off;
-      nseg_s = ns;
-    }
-    __syncthreads();
-    const int nseg = nseg_s;
-    // Stage the binding rows of every segment (same staged columns in every group).
-    for (int s = 0; s < nseg; ++s) {
-      const DevUGroup& G = U.g[seg_group[s]];
-      const int64_t src0 = seg_src[s];
-      const int n = seg_rows[s], dst0 = seg_off[s];
-      for (int c = 0; c <= ntc; ++c) {
-        const int32_t* src = G.tcols[c] + src0;
-        int32_t* dst = tstage + c * a.tile_rows + dst0;
-        for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
-      }
-    }
-
-    for (int s = 0; s < U.nsub; ++s) {
-      const DevSubset& S = U.sub[s];
-      __syncthreads();  // staging done / previous subset's probes done with table and lists
-      // Literal list of this subset, grouped by segment; entries per segment.
-      if (threadIdx.x == 0) {
-        int n = 0, e = 0;
-        for (int sg = 0; sg < nseg; ++sg) {
-          const DevUGroup& G = U.g[seg_group[sg]];
-          slit_start[sg] = n;
-          ent_start[sg] = e;
-          int k = 0;
-          for (int l = G.lit_begin; l < G.lit_begin + G.nlit; ++l)
-            if (a.lits[l].subset == s && n < EV_MAXLU) {
-              slits[n++] = l;
-              ++k;
-            }
-          e += k * ((seg_rows[sg] + 63) >> 6);
-        }
-        slit_start[nseg] = n;
-        ent_start[nseg] = e;
-      }
-      const uint32_t* cnt;
-      const int32_t* keys;
-      uint32_t cap;
-      if (hq < 0) {
-        cap = table_cap_lds(np);
-        uint32_t* state = tbl;
-        uint32_t* c_ = tbl + cap;
-        for (uint32_t i = threadIdx.x; i < 2 * cap; i += EV_THREADS) tbl[i] = 0;
-        __syncthreads();
-        for (int64_t r = threadIdx.x; r < np; r += EV_THREADS) {
-          int32_t vals[EV_MAXK];
-          for (int k = 0; k < S.nk; ++k) vals[k] = U.pcols[S.pcol[k]][p0 + r];
-          const uint32_t slot = table_claim(state, cap, vals, S.nk, U.pcols, S.pcol, p0, r);
-          atomicAdd(&c_[slot], 1u);
-        }
-        __syncthreads();
-        int32_t* k_ = reinterpret_cast<int32_t*>(tbl + 2 * cap);
-        for (uint32_t sl = threadIdx.x; sl < cap; sl += EV_THREADS) {
-          const uint32_t st = state[sl];
-          if (st)
-            for (int k = 0; k < S.nk; ++k) k_[(size_t)k * cap + sl] = U.pcols[S.pcol[k]][p0 + st - 1];
-        }
-        cnt = c_;
-        keys = k_;
-      } else {
-        cap = table_cap_hbm(np);
-        const uint32_t* base = S.hbuf + S.hbase[hq];
-        cnt = base + cap;
-        keys = reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap);
-      }
-      __syncthreads();  // table and lists ready
-
-      // Waves split the literal-major (literal, 64-row block) entries into contiguous ranges.
-      const int E = ent_start[nseg];
-      const int e0 = (int)((int64_t)E * wave / EV_WAVES), e1 = (int)((int64_t)E * (wave + 1) / EV_WAVES);
-      int sg = 0;
-      int cur_lit = -1;
-      uint64_t bp = 0, bn = 0, cp = 0, cn = 0;
-      for (int e = e0; e < e1; ++e) {
-        while (e >= ent_start[sg + 1]) ++sg;
-        const int nblk = (seg_rows[sg] + 63) >> 6;
-        const int local = e - ent_start[sg];
-        const int li = slits[slit_start[sg] + local / nblk];
-        const int blk = local % nblk;
-        if (li != cur_lit) {
-          if (cur_lit >= 0 && lane == 0) {
-            unsigned long long* out = a.counts_out + (int64_t)a.lits[cur_lit].slot * 4;
-            if (cp) atomicAdd(&out[0], (unsigned long long)cp);
-            if (cn) atomicAdd(&out[1], (unsigned long long)cn);
-            if (bp) atomicAdd(&out[2], (unsigned long long)bp);
-            if (bn) atomicAdd(&out[3], (unsigned long long)bn);
-          }
-          cur_lit = li;
-          bp = bn = cp = cn = 0;
-        }
-        const DevLit& L = a.lits[li];
-        const int rloc = blk * 64 + lane;
-        const bool valid = rloc < seg_rows[sg];
-        const int row = seg_off[sg] + rloc;
-        uint32_t c = 0;
-        int32_t tid = 0;
-        if (valid) {
-          bool ok = true;
-          for (int q2 = 0; q2 < L.neq; ++q2)
-            ok = ok && tstage[L.eq[q2][0] * a.tile_rows + row] == tstage[L.eq[q2][1] * a.tile_rows + row];
-          if (ok) {
-            int32_t vals[EV_MAXK];
-            for (int k = 0; k < L.nk; ++k) vals[k] = tstage[L.tcol[k] * a.tile_rows + row];
-            c = table_probe(cnt, keys, cap, vals, L.nk);
-          }
-          tid = tstage[ntc * a.tile_rows + row];
-        }
-        const bool pos = tid < a.num_positive;
-        const uint64_t mp = __ballot(c != 0 && pos), mn = __ballot(c != 0 && !pos);
-        cp += __popcll(mp);
-        cn += __popcll(mn);
-        if (__ballot(c > 1)) {
-          bp += wave_sum_u64(pos ? (uint64_t)c : 0ull);
-          bn += wave_sum_u64(pos ? 0ull : (uint64_t)c);
-        } else {
-          bp += __popcll(mp);
-          bn += __popcll(mn);
-        }
-        if (a.bits && c) atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
-      }
+  int sg = 0;
+  int cur_lit = -1;
+  uint64_t bp = 0, bn = 0, cp = 0, cn = 0;
+  for (int e = e0; e < e1; ++e) {
+    while (e >= sh.ent_start[sg + 1]) ++sg;
+    const int nblk = (sh.seg_rows[sg] + 63) >> 6;
+    const int local = e - sh.ent_start[sg];
+    const int li = __builtin_amdgcn_readfirstlane(U.g[sh.seg_group[sg]].sbeg[s] + local / nblk);
+    const int blk = local % nblk;
+    if (li != cur_lit) {
       if (cur_lit >= 0 && lane == 0) {
         unsigned long long* out = a.counts_out + (int64_t)a.lits[cur_lit].slot * 4;
         if (cp) atomicAdd(&out[0], (unsigned long long)cp);
         if (cn) atomicAdd(&out[1], (unsigned long long)cn);
         if (bp) atomicAdd(&out[2], (unsigned long long)bp);
         if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+      }
+      cur_lit = li;
+      bp = bn = cp = cn = 0;
+    }
+    const DevLit& L = a.lits[li];
+    const int rloc = blk * 64 + lane;
+    const bool valid = rloc < sh.seg_rows[sg];
+    const int row = sh.seg_off[sg] + rloc;
+    uint32_t c = 0;
+    int32_t tid = 0;
+    if (valid) {
+      bool ok = true;
+      for (int q2 = 0; q2 < L.neq; ++q2)
+        ok = ok && tstage[L.eq[q2][0] * a.tile_rows + row] == tstage[L.eq[q2][1] * a.tile_rows + row];
+      if (ok) {
+        int32_t vals[EV_MAXK];
+        for (int k = 0; k < L.nk; ++k) vals[k] = tstage[L.tcol[k] * a.tile_rows + row];
+        c = table_probe(cnt, keys, cap, vals, L.nk);
+      }
+      tid = tstage[ntc * a.tile_rows + row];
+    }
+    const bool pos = tid < a.num_positive;
+    const uint64_t mp = __ballot(c != 0 && pos), mn = __ballot(c != 0 && !pos);
+    cp += __popcll(mp);
+    cn += __popcll(mn);
+    if (__ballot(c > 1)) {
+      bp += wave_sum_u64(pos ? (uint64_t)c : 0ull);
+      bn += wave_sum_u64(pos ? 0ull : (uint64_t)c);
+    } else {
+      bp += __popcll(mp);
+      bn += __popcll(mn);
+    }
+    if (a.bits && c) atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+  }
+  if (cur_lit >= 0 && lane == 0) {
+    unsigned long long* out = a.counts_out + (int64_t)a.lits[cur_lit].slot * 4;
+    if (cp) atomicAdd(&out[0], (unsigned long long)cp);
+    if (cn) atomicAdd(&out[1], (unsigned long long)cn);
+    if (bp) atomicAdd(&out[2], (unsigned long long)bp);
+    if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+  }
+}
+
+__global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* tbl = lds;                                                                 // EV_TBL_WORDS
+  int32_t* pstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS);                   // EV_PST_WORDS
+  int32_t* tstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS + EV_PST_WORDS);   // (ntc + 1) x tile
+  __shared__ ItemShared sh;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t total = *a.num_items;
+  const int ntc = a.ntc;
+
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const DevItem I = a.items[it];
+    const DevUnit& U = a.units[I.unit];
+    const int64_t q = I.q;
+    const int64_t p0 = U.poff[q];
+    const int np = (int)(U.poff[q + 1] - p0);
+    const int hq = U.hq[q];
+
+    __syncthreads();  // the previous item is done with LDS
+    // Segments: the chunk's rows over the unit's join variables, in group order.
+    if (threadIdx.x == 0) {
+      const int64_t c0 = (int64_t)I.chunk * a.tile_rows, c1 = c0 + a.tile_rows;
+      int64_t run = 0;
+      int ns = 0, off = 0;
+      for (int g = 0; g < U.ngroups; ++g) {
+        const int64_t t0 = U.g[g].toff[q << U.shift], t1 = U.g[g].toff[(q + 1) << U.shift];
+        const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
+        if (hi_ > lo_) {
+          sh.seg_group[ns] = g;
+          sh.seg_off[ns] = off;
+          sh.seg_rows[ns] = (int)(hi_ - lo_);
+          sh.seg_src[ns] = t0 + (lo_ - run);
+          off += (int)(hi_ - lo_);
+          ++ns;
+        }
+        run += t1 - t0;
+      }
+      sh.seg_off[ns] = off;
+      sh.nseg = ns;
+    }
+    __syncthreads();
+    const int nseg = sh.nseg;
+    // Stage the binding rows of every segment (same staged columns in every group) and, for an
+    // LDS table, the partition's facts (all relation columns).
+    for (int sgi = 0; sgi < nseg; ++sgi) {
+      const DevUGroup& G = U.g[sh.seg_group[sgi]];
+      const int64_t src0 = sh.seg_src[sgi];
+      const int n = sh.seg_rows[sgi], dst0 = sh.seg_off[sgi];
+      for (int c = 0; c <= ntc; ++c) {
+        const int32_t* src = G.tcols[c] + src0;
+        int32_t* dst = tstage + c * a.tile_rows + dst0;
+        for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
+      }
+    }
+    if (hq < 0)
+      for (int c = 0; c < U.arity; ++c)
+        for (int i = threadIdx.x; i < np; i += EV_THREADS) pstage[c * np + i] = U.pcols[c][p0 + i];
+
+    for (int s = 0; s < U.nsub; ++s) {
+      const DevSubset& S = U.sub[s];
+      __syncthreads();  // staging done / previous subset's probes done with the table
+      if (threadIdx.x == 0) {
+        int e = 0;
+        for (int sg2 = 0; sg2 < nseg; ++sg2) {
+          const DevUGroup& G = U.g[sh.seg_group[sg2]];
+          sh.ent_start[sg2] = e;
+          e += (G.sbeg[s + 1] - G.sbeg[s]) * ((sh.seg_rows[sg2] + 63) >> 6);
+        }
+        sh.ent_start[nseg] = e;
+      }
+      if (hq < 0) {
+        const uint32_t cap = table_cap_lds(np);
+        uint32_t* state = tbl;
+        uint32_t* c_ = tbl + cap;
+        int32_t* k_ = reinterpret_cast<int32_t*>(tbl + 2 * cap);
+        for (uint32_t i = threadIdx.x; i < 2 * cap; i += EV_THREADS) tbl[i] = 0;
+        __syncthreads();
+        for (int r = threadIdx.x; r < np; r += EV_THREADS) {
+          int32_t vals[EV_MAXK];
+          for (int k = 0; k < S.nk; ++k) vals[k] = pstage[S.pcol[k] * np + r];
+          uint32_t slot = tuple_hash(vals, S.nk) & (cap - 1);
+          while (true) {
+            const uint32_t prev = atomicCAS(&state[slot], 0u, (uint32_t)(r + 1));
+            if (prev == 0) break;
+            bool eq = true;
+            for (int k = 0; k < S.nk; ++k) eq = eq && (pstage[S.pcol[k] * np + prev - 1] == vals[k]);
+            if (eq) break;
+            slot = (slot + 1) & (cap - 1);
+          }
+          atomicAdd(&c_[slot], 1u);
+        }
+        __syncthreads();
+        for (uint32_t sl = threadIdx.x; sl < cap; sl += EV_THREADS) {
+          const uint32_t st = state[sl];
+          if (st)
+            for (int k = 0; k < S.nk; ++k) k_[(size_t)k * cap + sl] = pstage[S.pcol[k] * np + st - 1];
+        }
+        __syncthreads();  // table and entry list ready
+        probe_entries<true>(a, U, sh, s, tstage, c_, k_, cap, lane, wave);
+      } else {
+        const uint32_t cap = table_cap_hbm(np);
+        const uint32_t* base = S.hbuf + S.hbase[hq];
+        __syncthreads();  // entry list ready
+        probe_entries<false>(a, U, sh, s, tstage, base + cap, reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap),
+                             cap, lane, wave);
       }
     }
   }
@@ -523,12 +550,13 @@ static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table&
   pt->h_offsets.resize(((size_t)1 << R) + 1);
   copy_d2h(ctx, pt->h_offsets.data(), pt->offsets.get(), pt->h_offsets.size() * 8);
   rp->pt = pt;
-  // Heavy: the LDS table of the widest possible key (all columns) does not fit.
+  // Heavy: the facts or the LDS table of the widest possible key (all columns) do not fit.
   const int64_t parts = (int64_t)1 << R;
   std::vector<int32_t> hq(parts, -1);
   for (int64_t q = 0; q < parts; ++q) {
     const int64_t np = pt->h_offsets[q + 1] - pt->h_offsets[q];
-    if (np > 0 && (int64_t)table_cap_lds(np) * (rel.num_columns + 2) > EV_TBL_WORDS) {
+    if (np > 0 && ((int64_t)table_cap_lds(np) * (rel.num_columns + 2) > EV_TBL_WORDS ||
+                   np * rel.num_columns > EV_PST_WORDS)) {
       hq[q] = (int32_t)rp->heavy_q.size();
       rp->heavy_q.push_back(q);
     }
@@ -626,8 +654,8 @@ struct StepPlan {
   std::vector<DevLit> lits;
   std::vector<std::pair<int, int>> launches;  // unit ranges launched together
   std::vector<int64_t> unit_cells;            // per launch-local prefix, concatenated per launch
-  DevBuf d_units, d_lits, d_unit_cells, d_cell_counts, d_cell_starts, d_counts;
-  int64_t max_cells = 0;
+  DevBuf d_units, d_lits, d_unit_cells, d_cell_counts, d_cell_starts, d_items, d_num_items, d_counts;
+  int64_t max_cells = 0, max_items = 0;
   size_t lds_bytes = 0;
   int max_blocks = 0;
   int num_slots = 0;
@@ -731,32 +759,39 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     du.shift = P.R_T - du.R;
     du.ngroups = (int)S.groups.size();
     QF_REQUIRE(du.ngroups <= EV_MAXG, "too many join variables in one unit");
+    du.arity = rp.pt->num_columns;
+    // Key subsets of the unit, in first-use order.
     std::vector<std::vector<int>> subsets;
+    for (int gi = 0; gi < du.ngroups; ++gi)
+      for (const LitSpec* l : S.lits[gi])
+        if (std::find(subsets.begin(), subsets.end(), l->pcols) == subsets.end()) subsets.push_back(l->pcols);
+    QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one unit");
     int unit_lits = 0;
     for (int gi = 0; gi < du.ngroups; ++gi) {
       const int vi = var_index(S.groups[gi]->join_var);
       DevUGroup& G = du.g[gi];
       for (size_t c = 0; c <= P.bcols.size(); ++c) G.tcols[c] = P.tparts[vi][c].as<int32_t>();
       G.toff = P.toffs[vi].as<int64_t>();
-      G.lit_begin = (int)P.lits.size();
-      for (const LitSpec* l : S.lits[gi]) {
-        int s = (int)(std::find(subsets.begin(), subsets.end(), l->pcols) - subsets.begin());
-        if (s == (int)subsets.size()) subsets.push_back(l->pcols);
-        QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one unit");
-        DevLit dl{};
-        dl.slot = l->slot;
-        dl.subset = s;
-        dl.nk = (int)l->pcols.size();
-        for (int k = 0; k < dl.nk; ++k) dl.tcol[k] = tidx(l->bcols[k]);
-        dl.neq = (int)l->eqs.size();
-        for (int e = 0; e < dl.neq; ++e) {
-          dl.eq[e][0] = tidx(l->eqs[e].first);
-          dl.eq[e][1] = tidx(l->eqs[e].second);
+      // The group's literals ordered by subset; sbeg gives each subset's range.
+      for (int s = 0; s < (int)subsets.size(); ++s) {
+        G.sbeg[s] = (int)P.lits.size();
+        for (const LitSpec* l : S.lits[gi]) {
+          if (l->pcols != subsets[s]) continue;
+          DevLit dl{};
+          dl.slot = l->slot;
+          dl.subset = s;
+          dl.nk = (int)l->pcols.size();
+          for (int k = 0; k < dl.nk; ++k) dl.tcol[k] = tidx(l->bcols[k]);
+          dl.neq = (int)l->eqs.size();
+          for (int e = 0; e < dl.neq; ++e) {
+            dl.eq[e][0] = tidx(l->eqs[e].first);
+            dl.eq[e][1] = tidx(l->eqs[e].second);
+          }
+          P.lits.push_back(dl);
+          ++unit_lits;
         }
-        P.lits.push_back(dl);
       }
-      G.nlit = (int)P.lits.size() - G.lit_begin;
-      unit_lits += G.nlit;
+      for (int s = (int)subsets.size(); s <= EV_MAXSUB; ++s) G.sbeg[s] = (int)P.lits.size();
     }
     QF_REQUIRE(unit_lits <= EV_MAXLU, "too many literals in one unit");
     du.nsub = (int)subsets.size();
@@ -781,13 +816,16 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     for (int u = 0; u < U; ++u) P.launches.emplace_back(u, u + 1);
   }
   for (auto& lr : P.launches) {
-    int64_t c = 0;
+    int64_t c = 0, items = 0;
     for (int u = lr.first; u < lr.second; ++u) {
       P.unit_cells.push_back(c);
       c += (int64_t)1 << P.units[u].R;
+      // Items of a unit: at most one per cell plus one per tile of its binding rows.
+      items += ((int64_t)1 << P.units[u].R) + (int64_t)P.units[u].ngroups * ceil_div(binding.num_rows, P.tile);
     }
     P.unit_cells.push_back(c);
     P.max_cells = std::max(P.max_cells, c);
+    P.max_items = std::max(P.max_items, items);
   }
   P.d_units = DevBuf(ctx, std::max<size_t>(1, P.units.size()) * sizeof(DevUnit));
   P.d_lits = DevBuf(ctx, std::max<size_t>(1, P.lits.size()) * sizeof(DevLit));
@@ -797,8 +835,10 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   if (!P.unit_cells.empty()) copy_h2d(ctx, P.d_unit_cells.get(), P.unit_cells.data(), P.unit_cells.size() * 8);
   P.d_cell_counts = DevBuf(ctx, std::max<int64_t>(1, P.max_cells) * 8);
   P.d_cell_starts = DevBuf(ctx, std::max<int64_t>(1, P.max_cells) * 8);
+  P.d_items = DevBuf(ctx, std::max<int64_t>(1, P.max_items) * sizeof(DevItem));
+  P.d_num_items = DevBuf(ctx, 8);
   P.d_counts = DevBuf(ctx, std::max(1, num_slots) * 4 * sizeof(uint64_t));
-  P.lds_bytes = (EV_TBL_WORDS + (size_t)(P.bcols.size() + 1) * P.tile) * 4;
+  P.lds_bytes = (EV_TBL_WORDS + EV_PST_WORDS + (size_t)(P.bcols.size() + 1) * P.tile) * 4;
   QF_REQUIRE(P.lds_bytes <= 160 * 1024 - 4096, "work item does not fit in LDS");
   QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)P.lds_bytes));
@@ -825,23 +865,27 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
     a.bits = reinterpret_cast<unsigned long long*>(d_bits);
     a.bit_words = P.bit_words;
     a.tile_rows = P.tile;
-    a.starts = P.d_cell_starts.as<int64_t>();
-    a.counts = P.d_cell_counts.as<int64_t>();
+    a.items = P.d_items.as<DevItem>();
+    a.num_items = P.d_num_items.as<int64_t>();
     size_t uc = 0;
     for (auto& lr : P.launches) {
       const int nu = lr.second - lr.first;
       const int64_t cells = P.unit_cells[uc + nu];
       a.units = P.d_units.as<DevUnit>() + lr.first;
-      a.unit_cells = P.d_unit_cells.as<int64_t>() + uc;
-      a.num_units = nu;
+      const int64_t* unit_cells = P.d_unit_cells.as<int64_t>() + uc;
       uc += nu + 1;
       if (cells == 0) continue;
       {
         PhaseScope ps(ctx, "work_items");
-        hipLaunchKernelGGL(k_count_items, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, ctx->stream, a.units,
-                           a.unit_cells, nu, P.tile, P.d_cell_counts.as<int64_t>());
+        const dim3 grid((unsigned)ceil_div(cells, 256));
+        hipLaunchKernelGGL(k_count_items, grid, dim3(256), 0, ctx->stream, a.units, unit_cells, nu, P.tile,
+                           P.d_cell_counts.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_cell_counts.as<int64_t>(), P.d_cell_starts.as<int64_t>(), cells);
+        hipLaunchKernelGGL(k_expand_items, grid, dim3(256), 0, ctx->stream, unit_cells, nu,
+                           P.d_cell_counts.as<int64_t>(), P.d_cell_starts.as<int64_t>(), P.d_items.as<DevItem>(),
+                           P.d_num_items.as<int64_t>());
+        QF_CHECK_HIP(hipGetLastError());
       }
       const int blocks = (int)std::min<int64_t>(P.max_blocks, std::max<int64_t>(1, cells));
       PhaseScope ps(ctx, "probe_count");
