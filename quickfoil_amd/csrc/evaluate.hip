@@ -33,7 +33,10 @@
 
 namespace qf {
 
-constexpr int EV_THREADS = 256;
+#ifndef QF_EV_THREADS
+#define QF_EV_THREADS 256
+#endif
+constexpr int EV_THREADS = QF_EV_THREADS;  // threads of one evaluation block (a multiple of 64)
 constexpr int EV_WAVES = EV_THREADS / 64;
 constexpr int EV_MAXK = QF_MAX_ARITY;  // key columns of one subset tuple (distinct relation columns)
 constexpr int EV_MAXEQ = 4;            // binding-side equalities of one literal (repeated columns)
@@ -80,7 +83,7 @@ struct DevUnit {
   DevSubset sub[EV_MAXSUB];
 };
 
-// Table layout (cap slots): state[cap] | cnt[cap] | keys[nk][cap].  state = 1 + partition-
+// HBM table layout (cap slots): state[cap] | cnt[cap] | keys[nk][cap].  state = 1 + partition-
 // relative row that claimed the slot; empty slots have cnt == 0.
 __device__ inline uint32_t tuple_hash(const int32_t* v, int nk) {
   uint32_t h = 0x2545f491u;
@@ -88,8 +91,11 @@ __device__ inline uint32_t tuple_hash(const int32_t* v, int nk) {
   return mix32(h);
 }
 
+// LDS tables: cap = P2(4 * facts) slots (load factor <= 1/4, so a miss visits ~1.4 slots).
+// slot word = (16-bit fingerprint << 16) | (1 + staged fact row), 0 = empty; count[cap] beside.
+constexpr int EV_LDS_MAX_FACTS = 1024;
 __host__ __device__ inline uint32_t table_cap_lds(int64_t np) {
-  uint32_t c = next_pow2_u32((uint32_t)((np * 4 + 2) / 3));
+  uint32_t c = next_pow2_u32((uint32_t)(4 * np));
   return c < EV_MIN_CAP ? EV_MIN_CAP : c;
 }
 __host__ __device__ inline uint32_t table_cap_hbm(int64_t np) {
@@ -264,13 +270,33 @@ struct ItemShared {
 
 // Probes the current subset's table for the (literal, 64-row block) entries of this wave.
 // kLds: the table is in LDS (the pointers come from the block's LDS array); else in HBM.
+// Probe of an LDS table: slot words carry a fingerprint and the representative fact, whose key
+// columns are compared in the staged facts.
+__device__ inline uint32_t lds_probe(const uint32_t* slots, const uint32_t* cnt, uint32_t cap, const int32_t* pstage,
+                                     int np, const int* pcol, const int32_t* vals, int nk) {
+  const uint32_t h = tuple_hash(vals, nk);
+  const uint32_t fp = h >> 16;
+  uint32_t slot = h & (cap - 1);
+  while (true) {
+    const uint32_t w = slots[slot];
+    if (w == 0) return 0;
+    if ((w >> 16) == fp) {
+      const int r = (int)(w & 0xffffu) - 1;
+      bool eq = true;
+      for (int k = 0; k < nk; ++k) eq = eq && (pstage[pcol[k] * np + r] == vals[k]);
+      if (eq) return cnt[slot];
+    }
+    slot = (slot + 1) & (cap - 1);
+  }
+}
+
 template <bool kLds>
 __device__ inline void probe_entries(const EvalArgs& a, const DevUnit& U, const ItemShared& sh, int s,
                                      const int32_t* tstage, const uint32_t* cnt, const int32_t* keys, uint32_t cap,
-                                     int lane, int wave) {
+                                     const int32_t* pstage, int np, int lane, int wave) {
   const int nseg = sh.nseg;
   const int E = sh.ent_start[nseg];
-  const int e0 = (int)((int64_t)E * wave / (EV_THREADS / 64)), e1 = (int)((int64_t)E * (wave + 1) / (EV_THREADS / 64));
+  const int e0 = (int)((int64_t)E * wave / EV_WAVES), e1 = (int)((int64_t)E * (wave + 1) / EV_WAVES);
   const int ntc = a.ntc;
   int sg = 0;
   int cur_lit = -1;
@@ -305,7 +331,8 @@ __device__ inline void probe_entries(const EvalArgs& a, const DevUnit& U, const 
       if (ok) {
         int32_t vals[EV_MAXK];
         for (int k = 0; k < L.nk; ++k) vals[k] = tstage[L.tcol[k] * a.tile_rows + row];
-        c = table_probe(cnt, keys, cap, vals, L.nk);
+        if (kLds) c = lds_probe(reinterpret_cast<const uint32_t*>(keys), cnt, cap, pstage, np, U.sub[s].pcol, vals, L.nk);
+        else c = table_probe(cnt, keys, cap, vals, L.nk);
       }
       tid = tstage[ntc * a.tile_rows + row];
     }
@@ -405,39 +432,38 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
       }
       if (hq < 0) {
         const uint32_t cap = table_cap_lds(np);
-        uint32_t* state = tbl;
+        uint32_t* slots = tbl;
         uint32_t* c_ = tbl + cap;
-        int32_t* k_ = reinterpret_cast<int32_t*>(tbl + 2 * cap);
         for (uint32_t i = threadIdx.x; i < 2 * cap; i += EV_THREADS) tbl[i] = 0;
         __syncthreads();
         for (int r = threadIdx.x; r < np; r += EV_THREADS) {
           int32_t vals[EV_MAXK];
           for (int k = 0; k < S.nk; ++k) vals[k] = pstage[S.pcol[k] * np + r];
-          uint32_t slot = tuple_hash(vals, S.nk) & (cap - 1);
+          const uint32_t h = tuple_hash(vals, S.nk);
+          const uint32_t mine = ((h >> 16) << 16) | (uint32_t)(r + 1);
+          uint32_t slot = h & (cap - 1);
           while (true) {
-            const uint32_t prev = atomicCAS(&state[slot], 0u, (uint32_t)(r + 1));
+            const uint32_t prev = atomicCAS(&slots[slot], 0u, mine);
             if (prev == 0) break;
-            bool eq = true;
-            for (int k = 0; k < S.nk; ++k) eq = eq && (pstage[S.pcol[k] * np + prev - 1] == vals[k]);
-            if (eq) break;
+            if ((prev >> 16) == (h >> 16)) {
+              const int rep = (int)(prev & 0xffffu) - 1;
+              bool eq = true;
+              for (int k = 0; k < S.nk; ++k) eq = eq && (pstage[S.pcol[k] * np + rep] == vals[k]);
+              if (eq) break;
+            }
             slot = (slot + 1) & (cap - 1);
           }
           atomicAdd(&c_[slot], 1u);
         }
-        __syncthreads();
-        for (uint32_t sl = threadIdx.x; sl < cap; sl += EV_THREADS) {
-          const uint32_t st = state[sl];
-          if (st)
-            for (int k = 0; k < S.nk; ++k) k_[(size_t)k * cap + sl] = pstage[S.pcol[k] * np + st - 1];
-        }
         __syncthreads();  // table and entry list ready
-        probe_entries<true>(a, U, sh, s, tstage, c_, k_, cap, lane, wave);
+        probe_entries<true>(a, U, sh, s, tstage, c_, reinterpret_cast<const int32_t*>(slots), cap, pstage, np, lane,
+                            wave);
       } else {
         const uint32_t cap = table_cap_hbm(np);
         const uint32_t* base = S.hbuf + S.hbase[hq];
         __syncthreads();  // entry list ready
         probe_entries<false>(a, U, sh, s, tstage, base + cap, reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap),
-                             cap, lane, wave);
+                             cap, pstage, np, lane, wave);
       }
     }
   }
@@ -550,12 +576,12 @@ static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table&
   pt->h_offsets.resize(((size_t)1 << R) + 1);
   copy_d2h(ctx, pt->h_offsets.data(), pt->offsets.get(), pt->h_offsets.size() * 8);
   rp->pt = pt;
-  // Heavy: the facts or the LDS table of the widest possible key (all columns) do not fit.
+  // Heavy: the partition's facts or its LDS table do not fit.
   const int64_t parts = (int64_t)1 << R;
   std::vector<int32_t> hq(parts, -1);
   for (int64_t q = 0; q < parts; ++q) {
     const int64_t np = pt->h_offsets[q + 1] - pt->h_offsets[q];
-    if (np > 0 && ((int64_t)table_cap_lds(np) * (rel.num_columns + 2) > EV_TBL_WORDS ||
+    if (np > 0 && (np > EV_LDS_MAX_FACTS || 2 * (int64_t)table_cap_lds(np) > EV_TBL_WORDS ||
                    np * rel.num_columns > EV_PST_WORDS)) {
       hq[q] = (int32_t)rp->heavy_q.size();
       rp->heavy_q.push_back(q);
@@ -654,6 +680,7 @@ struct StepPlan {
   std::vector<DevLit> lits;
   std::vector<std::pair<int, int>> launches;  // unit ranges launched together
   std::vector<int64_t> unit_cells;            // per launch-local prefix, concatenated per launch
+  std::vector<int64_t> launch_items;          // per launch: upper bound of its work items
   DevBuf d_units, d_lits, d_unit_cells, d_cell_counts, d_cell_starts, d_items, d_num_items, d_counts;
   int64_t max_cells = 0, max_items = 0;
   size_t lds_bytes = 0;
@@ -826,6 +853,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     P.unit_cells.push_back(c);
     P.max_cells = std::max(P.max_cells, c);
     P.max_items = std::max(P.max_items, items);
+    P.launch_items.push_back(items);
   }
   P.d_units = DevBuf(ctx, std::max<size_t>(1, P.units.size()) * sizeof(DevUnit));
   P.d_lits = DevBuf(ctx, std::max<size_t>(1, P.lits.size()) * sizeof(DevLit));
@@ -842,7 +870,9 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   QF_REQUIRE(P.lds_bytes <= 160 * 1024 - 4096, "work item does not fit in LDS");
   QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)P.lds_bytes));
-  P.max_blocks = ctx->num_cus * std::max(1, (int)((160 * 1024) / (P.lds_bytes + 4096)));
+  int per_cu = 0;
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eval, EV_THREADS, P.lds_bytes));
+  P.max_blocks = ctx->num_cus * std::max(1, per_cu);
   P.bit_words = ceil_div(binding.num_rows, 64);
 }
 
@@ -867,8 +897,9 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
     a.tile_rows = P.tile;
     a.items = P.d_items.as<DevItem>();
     a.num_items = P.d_num_items.as<int64_t>();
-    size_t uc = 0;
+    size_t uc = 0, li = 0;
     for (auto& lr : P.launches) {
+      const int64_t items_bound = P.launch_items[li++];
       const int nu = lr.second - lr.first;
       const int64_t cells = P.unit_cells[uc + nu];
       a.units = P.d_units.as<DevUnit>() + lr.first;
@@ -887,7 +918,8 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
                            P.d_num_items.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
       }
-      const int blocks = (int)std::min<int64_t>(P.max_blocks, std::max<int64_t>(1, cells));
+      // Blocks loop over the items the device listed; more blocks than the bound would idle.
+      const int blocks = (int)std::min<int64_t>(P.max_blocks, std::max<int64_t>(1, items_bound));
       PhaseScope ps(ctx, "probe_count");
       hipLaunchKernelGGL(k_eval, dim3(blocks), dim3(EV_THREADS), P.lds_bytes, ctx->stream, a);
       QF_CHECK_HIP(hipGetLastError());
