@@ -34,7 +34,7 @@
 namespace qf {
 
 #ifndef QF_EV_THREADS
-#define QF_EV_THREADS 256
+#define QF_EV_THREADS 512
 #endif
 constexpr int EV_THREADS = QF_EV_THREADS;  // threads of one evaluation block (a multiple of 64)
 constexpr int EV_WAVES = EV_THREADS / 64;
@@ -44,7 +44,7 @@ constexpr int EV_MAXSUB = 32;          // key subsets per unit
 constexpr int EV_MAXTC = 12;           // staged binding columns (plus the row id)
 constexpr int EV_MAXG = EV_MAXTC;      // join variables per unit
 constexpr int EV_MAXLU = 512;          // literals per unit
-constexpr int EV_TBL_WORDS = 8192;     // LDS words for one count table (32 KB)
+constexpr int EV_TBL_WORDS = 4096;     // LDS words for one count table (16 KB)
 constexpr int EV_PST_WORDS = 3072;     // LDS words for the partition's facts (12 KB)
 constexpr int EV_MIN_CAP = 16;
 
@@ -91,9 +91,30 @@ __device__ inline uint32_t tuple_hash(const int32_t* v, int nk) {
   return mix32(h);
 }
 
+// Read-only kernel inputs (work list, units, literals) are read through the constant address
+// space, so uniform reads become scalar loads.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ inline cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+// Marks a block-uniform value (read from LDS or memory) as wave-uniform for the compiler.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uni64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // LDS tables: cap = P2(4 * facts) slots (load factor <= 1/4, so a miss visits ~1.4 slots).
-// slot word = (16-bit fingerprint << 16) | (1 + staged fact row), 0 = empty; count[cap] beside.
+// One word per slot: fingerprint (10 bits) << 22 | (1 + staged fact row) << 11 | count (11 bits);
+// 0 = empty.  Facts and counts stay below 2048 because an LDS partition holds at most 1024 facts.
 constexpr int EV_LDS_MAX_FACTS = 1024;
+constexpr uint32_t EV_FP_SHIFT = 22;
+constexpr uint32_t EV_REP_SHIFT = 11;
+constexpr uint32_t EV_FIELD_MASK = 0x7ffu;
+static_assert(EV_LDS_MAX_FACTS <= (int)EV_FIELD_MASK, "fact rows and counts must fit 11 bits");
 __host__ __device__ inline uint32_t table_cap_lds(int64_t np) {
   uint32_t c = next_pow2_u32((uint32_t)(4 * np));
   return c < EV_MIN_CAP ? EV_MIN_CAP : c;
@@ -268,73 +289,115 @@ struct ItemShared {
   int nseg;
 };
 
-// Probes the current subset's table for the (literal, 64-row block) entries of this wave.
-// kLds: the table is in LDS (the pointers come from the block's LDS array); else in HBM.
-// Probe of an LDS table: slot words carry a fingerprint and the representative fact, whose key
-// columns are compared in the staged facts.
-__device__ inline uint32_t lds_probe(const uint32_t* slots, const uint32_t* cnt, uint32_t cap, const int32_t* pstage,
-                                     int np, const int* pcol, const int32_t* vals, int nk) {
-  const uint32_t h = tuple_hash(vals, nk);
-  const uint32_t fp = h >> 16;
+// Probe of an LDS table (packed slot words); the representative fact's key columns are compared
+// in the staged facts (poff[k] = word offset of key column k there).
+template <int NK>
+__device__ inline uint32_t lds_probe(const uint32_t* tbl, uint32_t cap, const int32_t* pstage, const int* poff,
+                                     const int32_t* vals, int nk) {
+  const uint32_t h = tuple_hash(vals, NK ? NK : nk);
+  const uint32_t fp = h >> EV_FP_SHIFT;
   uint32_t slot = h & (cap - 1);
   while (true) {
-    const uint32_t w = slots[slot];
+    const uint32_t w = tbl[slot];
     if (w == 0) return 0;
-    if ((w >> 16) == fp) {
-      const int r = (int)(w & 0xffffu) - 1;
+    if ((w >> EV_FP_SHIFT) == fp) {
+      const int r = (int)((w >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1;
       bool eq = true;
-      for (int k = 0; k < nk; ++k) eq = eq && (pstage[pcol[k] * np + r] == vals[k]);
-      if (eq) return cnt[slot];
+#pragma unroll
+      for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+        if (NK || k < nk) eq = eq && (pstage[poff[k] + r] == vals[k]);
+      if (eq) return w & EV_FIELD_MASK;
     }
     slot = (slot + 1) & (cap - 1);
   }
 }
 
-template <bool kLds>
-__device__ inline void probe_entries(const EvalArgs& a, const DevUnit& U, const ItemShared& sh, int s,
-                                     const int32_t* tstage, const uint32_t* cnt, const int32_t* keys, uint32_t cap,
-                                     const int32_t* pstage, int np, int lane, int wave) {
-  const int nseg = sh.nseg;
-  const int E = sh.ent_start[nseg];
+// Wave-uniform registers of the literal being probed.
+struct LitRegs {
+  int slot;
+  int neq;
+  int tb[EV_MAXK];  // word offsets of the key's binding columns in the staged tile
+  int eqa[EV_MAXEQ], eqb[EV_MAXEQ];
+};
+
+template <int NK>
+__device__ inline void load_literal(cptr<DevLit> lits, int li, int nk, int tile, LitRegs& L) {
+  cptr<DevLit> p = lits + li;
+  L.slot = p->slot;
+  L.neq = p->neq;
+#pragma unroll
+  for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+    if (NK || k < nk) L.tb[k] = p->tcol[k] * tile;
+#pragma unroll
+  for (int q = 0; q < EV_MAXEQ; ++q)
+    if (q < L.neq) {
+      L.eqa[q] = p->eq[q][0] * tile;
+      L.eqb[q] = p->eq[q][1] * tile;
+    }
+}
+
+__device__ inline void flush_counts(const EvalArgs& a, int slot, uint64_t& cp, uint64_t& cn, uint64_t& bp,
+                                    uint64_t& bn, int lane) {
+  if (lane == 0) {
+    unsigned long long* out = a.counts_out + (int64_t)slot * 4;
+    if (cp) atomicAdd(&out[0], (unsigned long long)cp);
+    if (cn) atomicAdd(&out[1], (unsigned long long)cn);
+    if (bp) atomicAdd(&out[2], (unsigned long long)bp);
+    if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+  }
+  cp = cn = bp = bn = 0;
+}
+
+// Probes the current subset's table for this wave's share of the (literal, 64-row block)
+// entries, which run literal-major within each segment.  kLds: packed LDS table `tbl`; else the
+// HBM table (cnt, keys).  NK > 0 fixes the key width; NK == 0 reads it from `nk`.
+template <int NK, bool kLds>
+__device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk,
+                              const int32_t* tstage, const uint32_t* tbl, const uint32_t* cnt, const int32_t* keys,
+                              uint32_t cap, const int32_t* pstage, int np, int lane, int wave) {
+  const int nseg = uni(sh.nseg);
+  const int E = uni(sh.ent_start[nseg]);
   const int e0 = (int)((int64_t)E * wave / EV_WAVES), e1 = (int)((int64_t)E * (wave + 1) / EV_WAVES);
-  const int ntc = a.ntc;
+  if (e0 >= e1) return;
+  const int tile = a.tile_rows;
+  const int32_t* tid_col = tstage + a.ntc * tile;
+  cptr<DevLit> lits = as_const(a.lits);
+  int poff[EV_MAXK];
+#pragma unroll
+  for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+    if (NK || k < nk) poff[k] = U->sub[s].pcol[k] * np;
+
+  // Position of entry e0: segment, literal within the segment, 64-row block.
   int sg = 0;
-  int cur_lit = -1;
+  while (e0 >= uni(sh.ent_start[sg + 1])) ++sg;
+  int seg_rows = uni(sh.seg_rows[sg]), seg_off = uni(sh.seg_off[sg]), seg_end = uni(sh.ent_start[sg + 1]);
+  int nblk = (seg_rows + 63) >> 6;
+  int lit = (e0 - uni(sh.ent_start[sg])) / nblk;
+  int blk = (e0 - uni(sh.ent_start[sg])) - lit * nblk;
+  int lit0 = U->g[uni(sh.seg_group[sg])].sbeg[s];
+  LitRegs L;
+  load_literal<NK>(lits, lit0 + lit, nk, tile, L);
   uint64_t bp = 0, bn = 0, cp = 0, cn = 0;
   for (int e = e0; e < e1; ++e) {
-    while (e >= sh.ent_start[sg + 1]) ++sg;
-    const int nblk = (sh.seg_rows[sg] + 63) >> 6;
-    const int local = e - sh.ent_start[sg];
-    const int li = __builtin_amdgcn_readfirstlane(U.g[sh.seg_group[sg]].sbeg[s] + local / nblk);
-    const int blk = local % nblk;
-    if (li != cur_lit) {
-      if (cur_lit >= 0 && lane == 0) {
-        unsigned long long* out = a.counts_out + (int64_t)a.lits[cur_lit].slot * 4;
-        if (cp) atomicAdd(&out[0], (unsigned long long)cp);
-        if (cn) atomicAdd(&out[1], (unsigned long long)cn);
-        if (bp) atomicAdd(&out[2], (unsigned long long)bp);
-        if (bn) atomicAdd(&out[3], (unsigned long long)bn);
-      }
-      cur_lit = li;
-      bp = bn = cp = cn = 0;
-    }
-    const DevLit& L = a.lits[li];
     const int rloc = blk * 64 + lane;
-    const bool valid = rloc < sh.seg_rows[sg];
-    const int row = sh.seg_off[sg] + rloc;
+    const bool valid = rloc < seg_rows;
+    const int row = seg_off + rloc;
     uint32_t c = 0;
     int32_t tid = 0;
     if (valid) {
       bool ok = true;
-      for (int q2 = 0; q2 < L.neq; ++q2)
-        ok = ok && tstage[L.eq[q2][0] * a.tile_rows + row] == tstage[L.eq[q2][1] * a.tile_rows + row];
+#pragma unroll
+      for (int q = 0; q < EV_MAXEQ; ++q)
+        if (q < L.neq) ok = ok && tstage[L.eqa[q] + row] == tstage[L.eqb[q] + row];
       if (ok) {
         int32_t vals[EV_MAXK];
-        for (int k = 0; k < L.nk; ++k) vals[k] = tstage[L.tcol[k] * a.tile_rows + row];
-        if (kLds) c = lds_probe(reinterpret_cast<const uint32_t*>(keys), cnt, cap, pstage, np, U.sub[s].pcol, vals, L.nk);
-        else c = table_probe(cnt, keys, cap, vals, L.nk);
+#pragma unroll
+        for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+          if (NK || k < nk) vals[k] = tstage[L.tb[k] + row];
+        if (kLds) c = lds_probe<NK>(tbl, cap, pstage, poff, vals, nk);
+        else c = table_probe(cnt, keys, cap, vals, NK ? NK : nk);
       }
-      tid = tstage[ntc * a.tile_rows + row];
+      tid = tid_col[row];
     }
     const bool pos = tid < a.num_positive;
     const uint64_t mp = __ballot(c != 0 && pos), mn = __ballot(c != 0 && !pos);
@@ -348,13 +411,76 @@ __device__ inline void probe_entries(const EvalArgs& a, const DevUnit& U, const 
       bn += __popcll(mn);
     }
     if (a.bits && c) atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+    if (++blk == nblk) {  // the literal's run in this segment ends
+      blk = 0;
+      flush_counts(a, L.slot, cp, cn, bp, bn, lane);
+      ++lit;
+      if (e + 1 < e1) {
+        if (e + 1 >= seg_end) {
+          do ++sg;
+          while (e + 1 >= uni(sh.ent_start[sg + 1]));
+          lit = 0;
+          seg_rows = uni(sh.seg_rows[sg]);
+          seg_off = uni(sh.seg_off[sg]);
+          seg_end = uni(sh.ent_start[sg + 1]);
+          nblk = (seg_rows + 63) >> 6;
+          lit0 = U->g[uni(sh.seg_group[sg])].sbeg[s];
+        }
+        load_literal<NK>(lits, lit0 + lit, nk, tile, L);
+      }
+    }
   }
-  if (cur_lit >= 0 && lane == 0) {
-    unsigned long long* out = a.counts_out + (int64_t)a.lits[cur_lit].slot * 4;
-    if (cp) atomicAdd(&out[0], (unsigned long long)cp);
-    if (cn) atomicAdd(&out[1], (unsigned long long)cn);
-    if (bp) atomicAdd(&out[2], (unsigned long long)bp);
-    if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+  flush_counts(a, L.slot, cp, cn, bp, bn, lane);
+}
+
+template <int NK>
+__device__ void build_lds_table(uint32_t* tbl, uint32_t cap, const int32_t* pstage, const int* poff, int np, int nk) {
+  for (int r = threadIdx.x; r < np; r += EV_THREADS) {
+    int32_t vals[EV_MAXK];
+#pragma unroll
+    for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+      if (NK || k < nk) vals[k] = pstage[poff[k] + r];
+    const uint32_t h = tuple_hash(vals, NK ? NK : nk);
+    const uint32_t fp = h >> EV_FP_SHIFT;
+    const uint32_t mine = (fp << EV_FP_SHIFT) | ((uint32_t)(r + 1) << EV_REP_SHIFT);
+    uint32_t slot = h & (cap - 1);
+    while (true) {
+      const uint32_t prev = atomicCAS(&tbl[slot], 0u, mine);
+      if (prev == 0) break;
+      if ((prev >> EV_FP_SHIFT) == fp) {
+        const int rep = (int)((prev >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1;
+        bool eq = true;
+#pragma unroll
+        for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+          if (NK || k < nk) eq = eq && (pstage[poff[k] + rep] == vals[k]);
+        if (eq) break;
+      }
+      slot = (slot + 1) & (cap - 1);
+    }
+    atomicAdd(&tbl[slot], 1u);
+  }
+}
+
+// One key subset of an item: build its count table (LDS partitions; the table words were
+// cleared and the entry list written before the barrier that precedes this call), then probe.
+template <int NK>
+__device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk, int hq,
+                                   uint32_t* tbl, const int32_t* tstage, const int32_t* pstage, int np, int lane,
+                                   int wave) {
+  if (hq < 0) {
+    const uint32_t cap = table_cap_lds(np);
+    int poff[EV_MAXK];
+#pragma unroll
+    for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+      if (NK || k < nk) poff[k] = U->sub[s].pcol[k] * np;
+    build_lds_table<NK>(tbl, cap, pstage, poff, np, nk);
+    __syncthreads();  // table ready
+    probe_entries<NK, true>(a, U, sh, s, nk, tstage, tbl, nullptr, nullptr, cap, pstage, np, lane, wave);
+  } else {
+    const uint32_t cap = table_cap_hbm(np);
+    const uint32_t* base = U->sub[s].hbuf + U->sub[s].hbase[hq];
+    probe_entries<NK, false>(a, U, sh, s, nk, tstage, nullptr, base + cap,
+                             reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap), cap, pstage, np, lane, wave);
   }
 }
 
@@ -366,26 +492,27 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
   __shared__ ItemShared sh;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int64_t total = *a.num_items;
+  const int wave = uni(threadIdx.x >> 6);
+  const int64_t total = uni64(*a.num_items);
   const int ntc = a.ntc;
+  cptr<DevItem> items = as_const(a.items);
 
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
-    const DevItem I = a.items[it];
-    const DevUnit& U = a.units[I.unit];
-    const int64_t q = I.q;
-    const int64_t p0 = U.poff[q];
-    const int np = (int)(U.poff[q + 1] - p0);
-    const int hq = U.hq[q];
+    const int item_unit = items[it].unit, item_chunk = items[it].chunk;
+    const int64_t q = items[it].q;
+    cptr<DevUnit> U = as_const(a.units) + item_unit;
+    const int64_t p0 = U->poff[q];
+    const int np = (int)(U->poff[q + 1] - p0);
+    const int hq = U->hq[q];
 
     __syncthreads();  // the previous item is done with LDS
     // Segments: the chunk's rows over the unit's join variables, in group order.
     if (threadIdx.x == 0) {
-      const int64_t c0 = (int64_t)I.chunk * a.tile_rows, c1 = c0 + a.tile_rows;
+      const int64_t c0 = (int64_t)item_chunk * a.tile_rows, c1 = c0 + a.tile_rows;
       int64_t run = 0;
       int ns = 0, off = 0;
-      for (int g = 0; g < U.ngroups; ++g) {
-        const int64_t t0 = U.g[g].toff[q << U.shift], t1 = U.g[g].toff[(q + 1) << U.shift];
+      for (int g = 0; g < U->ngroups; ++g) {
+        const int64_t t0 = U->g[g].toff[q << U->shift], t1 = U->g[g].toff[(q + 1) << U->shift];
         const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
         if (hi_ > lo_) {
           sh.seg_group[ns] = g;
@@ -401,69 +528,46 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
       sh.nseg = ns;
     }
     __syncthreads();
-    const int nseg = sh.nseg;
+    const int nseg = uni(sh.nseg);
     // Stage the binding rows of every segment (same staged columns in every group) and, for an
     // LDS table, the partition's facts (all relation columns).
     for (int sgi = 0; sgi < nseg; ++sgi) {
-      const DevUGroup& G = U.g[sh.seg_group[sgi]];
-      const int64_t src0 = sh.seg_src[sgi];
-      const int n = sh.seg_rows[sgi], dst0 = sh.seg_off[sgi];
+      const int g = uni(sh.seg_group[sgi]);
+      const int64_t src0 = uni64(sh.seg_src[sgi]);
+      const int n = uni(sh.seg_rows[sgi]), dst0 = uni(sh.seg_off[sgi]);
       for (int c = 0; c <= ntc; ++c) {
-        const int32_t* src = G.tcols[c] + src0;
+        const int32_t* src = U->g[g].tcols[c] + src0;
         int32_t* dst = tstage + c * a.tile_rows + dst0;
         for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
       }
     }
     if (hq < 0)
-      for (int c = 0; c < U.arity; ++c)
-        for (int i = threadIdx.x; i < np; i += EV_THREADS) pstage[c * np + i] = U.pcols[c][p0 + i];
+      for (int c = 0; c < U->arity; ++c) {
+        const int32_t* src = U->pcols[c] + p0;
+        for (int i = threadIdx.x; i < np; i += EV_THREADS) pstage[c * np + i] = src[i];
+      }
 
-    for (int s = 0; s < U.nsub; ++s) {
-      const DevSubset& S = U.sub[s];
+    for (int s = 0; s < U->nsub; ++s) {
       __syncthreads();  // staging done / previous subset's probes done with the table
       if (threadIdx.x == 0) {
         int e = 0;
         for (int sg2 = 0; sg2 < nseg; ++sg2) {
-          const DevUGroup& G = U.g[sh.seg_group[sg2]];
+          const int g = sh.seg_group[sg2];
           sh.ent_start[sg2] = e;
-          e += (G.sbeg[s + 1] - G.sbeg[s]) * ((sh.seg_rows[sg2] + 63) >> 6);
+          e += (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_rows[sg2] + 63) >> 6);
         }
         sh.ent_start[nseg] = e;
       }
-      if (hq < 0) {
-        const uint32_t cap = table_cap_lds(np);
-        uint32_t* slots = tbl;
-        uint32_t* c_ = tbl + cap;
-        for (uint32_t i = threadIdx.x; i < 2 * cap; i += EV_THREADS) tbl[i] = 0;
-        __syncthreads();
-        for (int r = threadIdx.x; r < np; r += EV_THREADS) {
-          int32_t vals[EV_MAXK];
-          for (int k = 0; k < S.nk; ++k) vals[k] = pstage[S.pcol[k] * np + r];
-          const uint32_t h = tuple_hash(vals, S.nk);
-          const uint32_t mine = ((h >> 16) << 16) | (uint32_t)(r + 1);
-          uint32_t slot = h & (cap - 1);
-          while (true) {
-            const uint32_t prev = atomicCAS(&slots[slot], 0u, mine);
-            if (prev == 0) break;
-            if ((prev >> 16) == (h >> 16)) {
-              const int rep = (int)(prev & 0xffffu) - 1;
-              bool eq = true;
-              for (int k = 0; k < S.nk; ++k) eq = eq && (pstage[S.pcol[k] * np + rep] == vals[k]);
-              if (eq) break;
-            }
-            slot = (slot + 1) & (cap - 1);
-          }
-          atomicAdd(&c_[slot], 1u);
-        }
-        __syncthreads();  // table and entry list ready
-        probe_entries<true>(a, U, sh, s, tstage, c_, reinterpret_cast<const int32_t*>(slots), cap, pstage, np, lane,
-                            wave);
-      } else {
-        const uint32_t cap = table_cap_hbm(np);
-        const uint32_t* base = S.hbuf + S.hbase[hq];
-        __syncthreads();  // entry list ready
-        probe_entries<false>(a, U, sh, s, tstage, base + cap, reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap),
-                             cap, pstage, np, lane, wave);
+      if (hq < 0)
+        for (uint32_t i = threadIdx.x, cap = table_cap_lds(np); i < cap; i += EV_THREADS) tbl[i] = 0;
+      __syncthreads();  // entry list and cleared table ready
+      if (uni(sh.ent_start[nseg]) == 0) continue;  // no literal of this subset in the item's groups
+      const int nk = U->sub[s].nk;
+      switch (nk) {
+        case 1: eval_subset<1>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
+        case 2: eval_subset<2>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
+        case 3: eval_subset<3>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
+        default: eval_subset<0>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
       }
     }
   }
@@ -581,7 +685,7 @@ static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table&
   std::vector<int32_t> hq(parts, -1);
   for (int64_t q = 0; q < parts; ++q) {
     const int64_t np = pt->h_offsets[q + 1] - pt->h_offsets[q];
-    if (np > 0 && (np > EV_LDS_MAX_FACTS || 2 * (int64_t)table_cap_lds(np) > EV_TBL_WORDS ||
+    if (np > 0 && (np > EV_LDS_MAX_FACTS || (int64_t)table_cap_lds(np) > EV_TBL_WORDS ||
                    np * rel.num_columns > EV_PST_WORDS)) {
       hq[q] = (int32_t)rp->heavy_q.size();
       rp->heavy_q.push_back(q);
