@@ -272,7 +272,7 @@ struct EvalArgs {
   const DevItem* items;
   const int64_t* num_items;
   int ntc;                // staged binding columns (the row id follows)
-  int64_t num_positive;
+  int num_positive;  // binding rows are below 2^31
   unsigned long long* counts_out;  // num_slots x 4
   unsigned long long* bits;        // num_slots x bit_words, or null
   int64_t bit_words;
@@ -305,7 +305,7 @@ __device__ inline uint32_t lds_probe(const uint32_t* tbl, uint32_t cap, const in
       bool eq = true;
 #pragma unroll
       for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
-        if (NK || k < nk) eq = eq && (pstage[poff[k] + r] == vals[k]);
+        if (NK || k < nk) eq &= (pstage[poff[k] + r] == vals[k]);
       if (eq) return w & EV_FIELD_MASK;
     }
     slot = (slot + 1) & (cap - 1);
@@ -336,16 +336,21 @@ __device__ inline void load_literal(cptr<DevLit> lits, int li, int nk, int tile,
     }
 }
 
-__device__ inline void flush_counts(const EvalArgs& a, int slot, uint64_t& cp, uint64_t& cn, uint64_t& bp,
-                                    uint64_t& bn, int lane) {
-  if (lane == 0) {
-    unsigned long long* out = a.counts_out + (int64_t)slot * 4;
-    if (cp) atomicAdd(&out[0], (unsigned long long)cp);
-    if (cn) atomicAdd(&out[1], (unsigned long long)cn);
-    if (bp) atomicAdd(&out[2], (unsigned long long)bp);
-    if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+// Flushes a literal run: covered counts come from ballots (wave-uniform), binding counts from
+// per-lane sums of cnt (positives, negatives) reduced across the wave here.
+__device__ inline void flush_counts(const EvalArgs& a, int slot, uint64_t& cp, uint64_t& cn, uint64_t& lbp,
+                                    uint64_t& lbn, int lane) {
+  if (cp | cn) {  // no covered row means no binding either
+    const uint64_t bp = wave_sum_u64(lbp), bn = wave_sum_u64(lbn);
+    if (lane == 0) {
+      unsigned long long* out = a.counts_out + (int64_t)slot * 4;
+      if (cp) atomicAdd(&out[0], (unsigned long long)cp);
+      if (cn) atomicAdd(&out[1], (unsigned long long)cn);
+      if (bp) atomicAdd(&out[2], (unsigned long long)bp);
+      if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+    }
   }
-  cp = cn = bp = bn = 0;
+  cp = cn = lbp = lbn = 0;
 }
 
 // Probes the current subset's table for this wave's share of the (literal, 64-row block)
@@ -377,7 +382,7 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
   int lit0 = U->g[uni(sh.seg_group[sg])].sbeg[s];
   LitRegs L;
   load_literal<NK>(lits, lit0 + lit, nk, tile, L);
-  uint64_t bp = 0, bn = 0, cp = 0, cn = 0;
+  uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
   for (int e = e0; e < e1; ++e) {
     const int rloc = blk * 64 + lane;
     const bool valid = rloc < seg_rows;
@@ -386,9 +391,11 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
     int32_t tid = 0;
     if (valid) {
       bool ok = true;
+      if (L.neq > 0) {
 #pragma unroll
-      for (int q = 0; q < EV_MAXEQ; ++q)
-        if (q < L.neq) ok = ok && tstage[L.eqa[q] + row] == tstage[L.eqb[q] + row];
+        for (int q = 0; q < EV_MAXEQ; ++q)
+          if (q < L.neq) ok &= tstage[L.eqa[q] + row] == tstage[L.eqb[q] + row];
+      }
       if (ok) {
         int32_t vals[EV_MAXK];
 #pragma unroll
@@ -400,20 +407,14 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
       tid = tid_col[row];
     }
     const bool pos = tid < a.num_positive;
-    const uint64_t mp = __ballot(c != 0 && pos), mn = __ballot(c != 0 && !pos);
-    cp += __popcll(mp);
-    cn += __popcll(mn);
-    if (__ballot(c > 1)) {
-      bp += wave_sum_u64(pos ? (uint64_t)c : 0ull);
-      bn += wave_sum_u64(pos ? 0ull : (uint64_t)c);
-    } else {
-      bp += __popcll(mp);
-      bn += __popcll(mn);
-    }
+    cp += __popcll(__ballot(c != 0 && pos));
+    cn += __popcll(__ballot(c != 0 && !pos));
+    lbp += pos ? c : 0u;
+    lbn += pos ? 0u : c;
     if (a.bits && c) atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
     if (++blk == nblk) {  // the literal's run in this segment ends
       blk = 0;
-      flush_counts(a, L.slot, cp, cn, bp, bn, lane);
+      flush_counts(a, L.slot, cp, cn, lbp, lbn, lane);
       ++lit;
       if (e + 1 < e1) {
         if (e + 1 >= seg_end) {
@@ -430,7 +431,7 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
       }
     }
   }
-  flush_counts(a, L.slot, cp, cn, bp, bn, lane);
+  flush_counts(a, L.slot, cp, cn, lbp, lbn, lane);
 }
 
 template <int NK>
@@ -994,7 +995,7 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
     EvalArgs a{};
     a.lits = P.d_lits.as<DevLit>();
     a.ntc = (int)P.bcols.size();
-    a.num_positive = num_positive;
+    a.num_positive = (int)num_positive;
     a.counts_out = reinterpret_cast<unsigned long long*>(P.d_counts.get());
     a.bits = reinterpret_cast<unsigned long long*>(d_bits);
     a.bit_words = P.bit_words;
