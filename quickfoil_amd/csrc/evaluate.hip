@@ -44,9 +44,10 @@ constexpr int EV_MAXSUB = 32;          // key subsets per unit
 constexpr int EV_MAXTC = 12;           // staged binding columns (plus the row id)
 constexpr int EV_MAXG = EV_MAXTC;      // join variables per unit
 constexpr int EV_MAXLU = 512;          // literals per unit
-constexpr int EV_TBL_WORDS = 4096;     // LDS words for one count table (16 KB)
+constexpr int EV_TBL_WORDS = 2048;     // LDS words for one count table (8 KB)
 constexpr int EV_PST_WORDS = 3072;     // LDS words for the partition's facts (12 KB)
 constexpr int EV_MIN_CAP = 16;
+constexpr int EV_MAXMASK = 64;         // literals per unit whose hit masks prune other literals
 
 struct DevSubset {
   int nk;
@@ -62,12 +63,15 @@ struct DevLit {
   int neq;
   int tcol[EV_MAXK];    // staged binding column per key position
   int eq[EV_MAXEQ][2];  // staged binding columns that must be equal in the row
+  int pmask;            // hit-mask slot of a literal whose key is this key minus one column, or -1
+  int mslot;            // this literal's hit-mask slot (it prunes others), or -1
 };
 
 struct DevUGroup {
   const int32_t* tcols[EV_MAXTC + 1];  // binding table partitioned on this join variable; [ntc] = row id
   const int64_t* toff;
-  int sbeg[EV_MAXSUB + 1];             // literals of subset s: lits[sbeg[s], sbeg[s + 1])
+  int jtcol;                           // staged column of the join variable
+  int sbeg[EV_MAXSUB + 1];             // literals of subset s: lits[sbeg[s], sbeg[s + 1]); subset 0 = {join column}
 };
 
 struct DevUnit {
@@ -79,6 +83,7 @@ struct DevUnit {
   int arity;
   int ngroups;
   int nsub;
+  int jcol;  // the join column (subset 0's key)
   DevUGroup g[EV_MAXG];
   DevSubset sub[EV_MAXSUB];
 };
@@ -107,8 +112,8 @@ __device__ inline int64_t uni64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// LDS tables: cap = P2(4 * facts) slots (load factor <= 1/4, so a miss visits ~1.4 slots).
-// One word per slot: fingerprint (10 bits) << 22 | (1 + staged fact row) << 11 | count (11 bits);
+// LDS tables: cap = P2(4 * facts) slots (load factor <= 1/4, so a miss visits ~1.4 slots), or
+// P2(2 * facts) (load <= 1/2) when that would not fit the table region.  One word per slot: fingerprint (10 bits) << 22 | (1 + staged fact row) << 11 | count (11 bits);
 // 0 = empty.  Facts and counts stay below 2048 because an LDS partition holds at most 1024 facts.
 constexpr int EV_LDS_MAX_FACTS = 1024;
 constexpr uint32_t EV_FP_SHIFT = 22;
@@ -117,6 +122,7 @@ constexpr uint32_t EV_FIELD_MASK = 0x7ffu;
 static_assert(EV_LDS_MAX_FACTS <= (int)EV_FIELD_MASK, "fact rows and counts must fit 11 bits");
 __host__ __device__ inline uint32_t table_cap_lds(int64_t np) {
   uint32_t c = next_pow2_u32((uint32_t)(4 * np));
+  if (c > (uint32_t)EV_TBL_WORDS) c = next_pow2_u32((uint32_t)(2 * np));
   return c < EV_MIN_CAP ? EV_MIN_CAP : c;
 }
 __host__ __device__ inline uint32_t table_cap_hbm(int64_t np) {
@@ -277,6 +283,9 @@ struct EvalArgs {
   unsigned long long* bits;        // num_slots x bit_words, or null
   int64_t bit_words;
   int tile_rows;
+  int cidx_off;  // LDS word offsets: compacted row indices (uint16 x tile), hit masks (uint64)
+  int mask_off;
+  int mstride;   // 64-row blocks per mask slot (tile / 64)
 };
 
 // Per-item shared state, filled by the block's first threads.
@@ -285,7 +294,10 @@ struct ItemShared {
   int seg_off[EV_MAXG + 1];
   int seg_rows[EV_MAXG];
   int64_t seg_src[EV_MAXG];
-  int ent_start[EV_MAXG + 1];
+  int seg_hits[EV_MAXG];       // rows of the segment whose join value occurs in the partition
+  int blk_start[EV_MAXG + 1];  // join phase: 64-row blocks of all rows, per segment
+  int ent_start[EV_MAXG + 1];  // subset phase: (literal, block of hit rows) entries, per segment
+  int need[EV_MAXSUB];         // subset s has a literal whose parent mask is not all zero
   int nseg;
 };
 
@@ -316,6 +328,7 @@ __device__ inline uint32_t lds_probe(const uint32_t* tbl, uint32_t cap, const in
 struct LitRegs {
   int slot;
   int neq;
+  int pmask, mslot;
   int tb[EV_MAXK];  // word offsets of the key's binding columns in the staged tile
   int eqa[EV_MAXEQ], eqb[EV_MAXEQ];
 };
@@ -325,6 +338,8 @@ __device__ inline void load_literal(cptr<DevLit> lits, int li, int nk, int tile,
   cptr<DevLit> p = lits + li;
   L.slot = p->slot;
   L.neq = p->neq;
+  L.pmask = p->pmask;
+  L.mslot = p->mslot;
 #pragma unroll
   for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
     if (NK || k < nk) L.tb[k] = p->tcol[k] * tile;
@@ -336,35 +351,114 @@ __device__ inline void load_literal(cptr<DevLit> lits, int li, int nk, int tile,
     }
 }
 
-// Flushes a literal run: covered counts come from ballots (wave-uniform), binding counts from
-// per-lane sums of cnt (positives, negatives) reduced across the wave here.
-__device__ inline void flush_counts(const EvalArgs& a, int slot, uint64_t& cp, uint64_t& cn, uint64_t& lbp,
-                                    uint64_t& lbn, int lane) {
+// Adds a run's counts to literals [l0, l1) (several literals of one group can share a key):
+// covered counts come from ballots (wave-uniform), binding counts from per-lane sums of cnt
+// (positives, negatives) reduced across the wave here.
+__device__ inline void flush_counts(const EvalArgs& a, cptr<DevLit> lits, int l0, int l1, uint64_t& cp,
+                                    uint64_t& cn, uint64_t& lbp, uint64_t& lbn, int lane) {
   if (cp | cn) {  // no covered row means no binding either
     const uint64_t bp = wave_sum_u64(lbp), bn = wave_sum_u64(lbn);
-    if (lane == 0) {
-      unsigned long long* out = a.counts_out + (int64_t)slot * 4;
-      if (cp) atomicAdd(&out[0], (unsigned long long)cp);
-      if (cn) atomicAdd(&out[1], (unsigned long long)cn);
-      if (bp) atomicAdd(&out[2], (unsigned long long)bp);
-      if (bn) atomicAdd(&out[3], (unsigned long long)bn);
-    }
+    if (lane == 0)
+      for (int li = l0; li < l1; ++li) {
+        unsigned long long* out = a.counts_out + (int64_t)lits[li].slot * 4;
+        if (cp) atomicAdd(&out[0], (unsigned long long)cp);
+        if (cn) atomicAdd(&out[1], (unsigned long long)cn);
+        if (bp) atomicAdd(&out[2], (unsigned long long)bp);
+        if (bn) atomicAdd(&out[3], (unsigned long long)bn);
+      }
   }
   cp = cn = lbp = lbn = 0;
 }
 
-// Probes the current subset's table for this wave's share of the (literal, 64-row block)
-// entries, which run literal-major within each segment.  kLds: packed LDS table `tbl`; else the
-// HBM table (cnt, keys).  NK > 0 fixes the key width; NK == 0 reads it from `nk`.
+__device__ inline uint32_t lane_prefix(uint64_t m, int lane) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Join phase of an item: every staged row probes the table of the join column alone.  Rows
+// whose join value occurs in the partition are listed per segment in `cidx` (segment-relative
+// row numbers); every literal of the unit binds the join column, so the other rows match no
+// literal.  The literals keyed on the join column alone (subset 0) are counted here.
+template <bool kLds>
+__device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, const int32_t* tstage, uint16_t* cidx,
+                           const uint32_t* tbl, const uint32_t* cnt, const int32_t* keys, uint32_t cap,
+                           const int32_t* pstage, int np, int lane, int wave) {
+  const int nseg = uni(sh.nseg);
+  const int E = uni(sh.blk_start[nseg]);
+  const int e0 = (int)((int64_t)E * wave / EV_WAVES), e1 = (int)((int64_t)E * (wave + 1) / EV_WAVES);
+  if (e0 >= e1) return;
+  const int tile = a.tile_rows;
+  const int32_t* tid_col = tstage + a.ntc * tile;
+  cptr<DevLit> lits = as_const(a.lits);
+  const int poff = U->jcol * np;
+  int sg = 0;
+  while (e0 >= uni(sh.blk_start[sg + 1])) ++sg;
+  int blk = e0 - uni(sh.blk_start[sg]);
+  int seg_rows = uni(sh.seg_rows[sg]), seg_off = uni(sh.seg_off[sg]), nblk = (seg_rows + 63) >> 6;
+  int g = uni(sh.seg_group[sg]);
+  int jt = U->g[g].jtcol * tile, l0 = U->g[g].sbeg[0], l1 = U->g[g].sbeg[1];
+  uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
+  for (int e = e0; e < e1; ++e) {
+    const int rloc = blk * 64 + lane;
+    const bool valid = rloc < seg_rows;
+    const int row = seg_off + (valid ? rloc : 0);
+    uint32_t c = 0;
+    if (valid) {
+      const int32_t v = tstage[jt + row];
+      if (kLds) c = lds_probe<1>(tbl, cap, pstage, &poff, &v, 1);
+      else c = table_probe(cnt, keys, cap, &v, 1);
+    }
+    const uint64_t m = __ballot(c != 0);
+    if (m) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&sh.seg_hits[sg], __popcll(m));
+      base = __shfl(base, 0, 64);
+      if (c != 0) cidx[seg_off + base + (int)lane_prefix(m, lane)] = (uint16_t)rloc;
+      if (l1 > l0) {
+        const int32_t tid = tid_col[row];
+        const bool pos = tid < a.num_positive;
+        cp += __popcll(__ballot(c != 0 && pos));
+        cn += __popcll(__ballot(c != 0 && !pos));
+        lbp += pos ? c : 0u;
+        lbn += pos ? 0u : c;
+        if (a.bits != nullptr && c != 0)
+          for (int li = l0; li < l1; ++li)
+            atomicOr(&a.bits[(int64_t)lits[li].slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+      }
+    }
+    if (++blk == nblk) {  // the segment's blocks end
+      flush_counts(a, lits, l0, l1, cp, cn, lbp, lbn, lane);
+      blk = 0;
+      if (e + 1 < e1) {
+        ++sg;
+        seg_rows = uni(sh.seg_rows[sg]);
+        seg_off = uni(sh.seg_off[sg]);
+        nblk = (seg_rows + 63) >> 6;
+        g = uni(sh.seg_group[sg]);
+        jt = U->g[g].jtcol * tile;
+        l0 = U->g[g].sbeg[0];
+        l1 = U->g[g].sbeg[1];
+      }
+    }
+  }
+  flush_counts(a, lits, l0, l1, cp, cn, lbp, lbn, lane);
+}
+
+// Probes subset s's table for this wave's share of the (literal, 64-row block of join-hit
+// rows) entries, which run literal-major within each segment.  A literal with a parent mask
+// skips blocks where its parent matched no row and probes only rows where it did; a literal
+// with its own mask slot records its hit rows for its children.  kLds: packed LDS table `tbl`;
+// else the HBM table (cnt, keys).  NK > 0 fixes the key width; NK == 0 reads it from `nk`.
 template <int NK, bool kLds>
 __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk,
-                              const int32_t* tstage, const uint32_t* tbl, const uint32_t* cnt, const int32_t* keys,
-                              uint32_t cap, const int32_t* pstage, int np, int lane, int wave) {
+                              const int32_t* tstage, const uint16_t* cidx, uint64_t* masks, const uint32_t* tbl,
+                              const uint32_t* cnt, const int32_t* keys, uint32_t cap, const int32_t* pstage, int np,
+                              int lane, int wave) {
   const int nseg = uni(sh.nseg);
   const int E = uni(sh.ent_start[nseg]);
   const int e0 = (int)((int64_t)E * wave / EV_WAVES), e1 = (int)((int64_t)E * (wave + 1) / EV_WAVES);
   if (e0 >= e1) return;
   const int tile = a.tile_rows;
+  const int mstride = a.mstride;
   const int32_t* tid_col = tstage + a.ntc * tile;
   cptr<DevLit> lits = as_const(a.lits);
   int poff[EV_MAXK];
@@ -372,11 +466,11 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
   for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
     if (NK || k < nk) poff[k] = U->sub[s].pcol[k] * np;
 
-  // Position of entry e0: segment, literal within the segment, 64-row block.
+  // Position of entry e0: segment, literal within the segment, block of hit rows.
   int sg = 0;
   while (e0 >= uni(sh.ent_start[sg + 1])) ++sg;
-  int seg_rows = uni(sh.seg_rows[sg]), seg_off = uni(sh.seg_off[sg]), seg_end = uni(sh.ent_start[sg + 1]);
-  int nblk = (seg_rows + 63) >> 6;
+  int seg_hits = uni(sh.seg_hits[sg]), seg_off = uni(sh.seg_off[sg]), seg_end = uni(sh.ent_start[sg + 1]);
+  int nblk = (seg_hits + 63) >> 6;
   int lit = (e0 - uni(sh.ent_start[sg])) / nblk;
   int blk = (e0 - uni(sh.ent_start[sg])) - lit * nblk;
   int lit0 = U->g[uni(sh.seg_group[sg])].sbeg[s];
@@ -385,53 +479,63 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
   uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
   for (int e = e0; e < e1; ++e) {
     const int rloc = blk * 64 + lane;
-    const bool valid = rloc < seg_rows;
-    const int row = seg_off + rloc;
+    uint64_t pm = ~0ull;
+    if (L.pmask >= 0) pm = uni64((int64_t)masks[L.pmask * mstride + blk]);
     uint32_t c = 0;
-    int32_t tid = 0;
-    if (valid) {
-      bool ok = true;
-      if (L.neq > 0) {
+    if (pm != 0) {
+      const bool valid = rloc < seg_hits && ((pm >> lane) & 1);
+      const int row = seg_off + (rloc < seg_hits ? (int)cidx[seg_off + rloc] : 0);
+      if (valid) {
+        bool ok = true;
+        if (L.neq > 0) {
 #pragma unroll
-        for (int q = 0; q < EV_MAXEQ; ++q)
-          if (q < L.neq) ok &= tstage[L.eqa[q] + row] == tstage[L.eqb[q] + row];
-      }
-      if (ok) {
-        int32_t vals[EV_MAXK];
+          for (int q = 0; q < EV_MAXEQ; ++q)
+            if (q < L.neq) ok &= tstage[L.eqa[q] + row] == tstage[L.eqb[q] + row];
+        }
+        if (ok) {
+          int32_t vals[EV_MAXK];
 #pragma unroll
-        for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
-          if (NK || k < nk) vals[k] = tstage[L.tb[k] + row];
-        if (kLds) c = lds_probe<NK>(tbl, cap, pstage, poff, vals, nk);
-        else c = table_probe(cnt, keys, cap, vals, NK ? NK : nk);
+          for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
+            if (NK || k < nk) vals[k] = tstage[L.tb[k] + row];
+          if (kLds) c = lds_probe<NK>(tbl, cap, pstage, poff, vals, nk);
+          else c = table_probe(cnt, keys, cap, vals, NK ? NK : nk);
+        }
       }
-      tid = tid_col[row];
+      if (__ballot(c != 0)) {
+        const int32_t tid = tid_col[row];
+        const bool pos = tid < a.num_positive;
+        cp += __popcll(__ballot(c != 0 && pos));
+        cn += __popcll(__ballot(c != 0 && !pos));
+        lbp += pos ? c : 0u;
+        lbn += pos ? 0u : c;
+        if (a.bits != nullptr && c != 0)
+          atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+      }
     }
-    const bool pos = tid < a.num_positive;
-    cp += __popcll(__ballot(c != 0 && pos));
-    cn += __popcll(__ballot(c != 0 && !pos));
-    lbp += pos ? c : 0u;
-    lbn += pos ? 0u : c;
-    if (a.bits && c) atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+    if (L.mslot >= 0) {
+      const uint64_t hm = __ballot(c != 0);
+      if (lane == 0) masks[L.mslot * mstride + blk] = hm;
+    }
     if (++blk == nblk) {  // the literal's run in this segment ends
       blk = 0;
-      flush_counts(a, L.slot, cp, cn, lbp, lbn, lane);
+      flush_counts(a, lits, lit0 + lit, lit0 + lit + 1, cp, cn, lbp, lbn, lane);
       ++lit;
       if (e + 1 < e1) {
         if (e + 1 >= seg_end) {
           do ++sg;
           while (e + 1 >= uni(sh.ent_start[sg + 1]));
           lit = 0;
-          seg_rows = uni(sh.seg_rows[sg]);
+          seg_hits = uni(sh.seg_hits[sg]);
           seg_off = uni(sh.seg_off[sg]);
           seg_end = uni(sh.ent_start[sg + 1]);
-          nblk = (seg_rows + 63) >> 6;
+          nblk = (seg_hits + 63) >> 6;
           lit0 = U->g[uni(sh.seg_group[sg])].sbeg[s];
         }
         load_literal<NK>(lits, lit0 + lit, nk, tile, L);
       }
     }
   }
-  flush_counts(a, L.slot, cp, cn, lbp, lbn, lane);
+  flush_counts(a, lits, lit0 + lit, lit0 + lit + 1, cp, cn, lbp, lbn, lane);  // no-op after a completed run
 }
 
 template <int NK>
@@ -462,12 +566,12 @@ __device__ void build_lds_table(uint32_t* tbl, uint32_t cap, const int32_t* psta
   }
 }
 
-// One key subset of an item: build its count table (LDS partitions; the table words were
-// cleared and the entry list written before the barrier that precedes this call), then probe.
+// One key subset s >= 1 of an item: build its count table (LDS partitions; the table words were
+// cleared before the barrier that precedes this call), then probe.
 template <int NK>
 __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk, int hq,
-                                   uint32_t* tbl, const int32_t* tstage, const int32_t* pstage, int np, int lane,
-                                   int wave) {
+                                   uint32_t* tbl, const int32_t* tstage, const uint16_t* cidx, uint64_t* masks,
+                                   const int32_t* pstage, int np, int lane, int wave) {
   if (hq < 0) {
     const uint32_t cap = table_cap_lds(np);
     int poff[EV_MAXK];
@@ -476,11 +580,11 @@ __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const Ite
       if (NK || k < nk) poff[k] = U->sub[s].pcol[k] * np;
     build_lds_table<NK>(tbl, cap, pstage, poff, np, nk);
     __syncthreads();  // table ready
-    probe_entries<NK, true>(a, U, sh, s, nk, tstage, tbl, nullptr, nullptr, cap, pstage, np, lane, wave);
+    probe_entries<NK, true>(a, U, sh, s, nk, tstage, cidx, masks, tbl, nullptr, nullptr, cap, pstage, np, lane, wave);
   } else {
     const uint32_t cap = table_cap_hbm(np);
     const uint32_t* base = U->sub[s].hbuf + U->sub[s].hbase[hq];
-    probe_entries<NK, false>(a, U, sh, s, nk, tstage, nullptr, base + cap,
+    probe_entries<NK, false>(a, U, sh, s, nk, tstage, cidx, masks, nullptr, base + cap,
                              reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap), cap, pstage, np, lane, wave);
   }
 }
@@ -490,6 +594,8 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
   uint32_t* tbl = lds;                                                                 // EV_TBL_WORDS
   int32_t* pstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS);                   // EV_PST_WORDS
   int32_t* tstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS + EV_PST_WORDS);   // (ntc + 1) x tile
+  uint16_t* cidx = reinterpret_cast<uint16_t*>(lds + a.cidx_off);                    // tile
+  uint64_t* masks = reinterpret_cast<uint64_t*>(lds + a.mask_off);                   // slots x mstride
   __shared__ ItemShared sh;
 
   const int lane = threadIdx.x & 63;
@@ -497,6 +603,7 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
   const int64_t total = uni64(*a.num_items);
   const int ntc = a.ntc;
   cptr<DevItem> items = as_const(a.items);
+  cptr<DevLit> lits = as_const(a.lits);
 
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
     const int item_unit = items[it].unit, item_chunk = items[it].chunk;
@@ -511,22 +618,28 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
     if (threadIdx.x == 0) {
       const int64_t c0 = (int64_t)item_chunk * a.tile_rows, c1 = c0 + a.tile_rows;
       int64_t run = 0;
-      int ns = 0, off = 0;
+      int ns = 0, off = 0, blocks = 0;
       for (int g = 0; g < U->ngroups; ++g) {
         const int64_t t0 = U->g[g].toff[q << U->shift], t1 = U->g[g].toff[(q + 1) << U->shift];
         const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
         if (hi_ > lo_) {
+          const int n = (int)(hi_ - lo_);
           sh.seg_group[ns] = g;
           sh.seg_off[ns] = off;
-          sh.seg_rows[ns] = (int)(hi_ - lo_);
+          sh.seg_rows[ns] = n;
           sh.seg_src[ns] = t0 + (lo_ - run);
-          off += (int)(hi_ - lo_);
+          sh.seg_hits[ns] = 0;
+          sh.blk_start[ns] = blocks;
+          off += n;
+          blocks += (n + 63) >> 6;
           ++ns;
         }
         run += t1 - t0;
       }
       sh.seg_off[ns] = off;
+      sh.blk_start[ns] = blocks;
       sh.nseg = ns;
+      for (int s = 0; s < EV_MAXSUB; ++s) sh.need[s] = 0;
     }
     __syncthreads();
     const int nseg = uni(sh.nseg);
@@ -542,33 +655,61 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
         for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
       }
     }
-    if (hq < 0)
+    if (hq < 0) {
       for (int c = 0; c < U->arity; ++c) {
         const int32_t* src = U->pcols[c] + p0;
         for (int i = threadIdx.x; i < np; i += EV_THREADS) pstage[c * np + i] = src[i];
       }
+      for (uint32_t i = threadIdx.x, cap = table_cap_lds(np); i < cap; i += EV_THREADS) tbl[i] = 0;
+    }
+    __syncthreads();  // staged rows, facts and cleared table ready
 
-    for (int s = 0; s < U->nsub; ++s) {
-      __syncthreads();  // staging done / previous subset's probes done with the table
+    // Join phase (subset 0: the join column alone).
+    if (hq < 0) {
+      const uint32_t cap = table_cap_lds(np);
+      const int poff = U->jcol * np;
+      build_lds_table<1>(tbl, cap, pstage, &poff, np, 1);
+      __syncthreads();  // table ready
+      join_phase<true>(a, U, sh, tstage, cidx, tbl, nullptr, nullptr, cap, pstage, np, lane, wave);
+    } else {
+      const uint32_t cap = table_cap_hbm(np);
+      const uint32_t* base = U->sub[0].hbuf + U->sub[0].hbase[hq];
+      join_phase<false>(a, U, sh, tstage, cidx, nullptr, base + cap,
+                        reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap), cap, pstage, np, lane, wave);
+    }
+
+    for (int s = 1; s < U->nsub; ++s) {
+      __syncthreads();  // hit lists / previous subset's probes and masks done
       if (threadIdx.x == 0) {
         int e = 0;
         for (int sg2 = 0; sg2 < nseg; ++sg2) {
           const int g = sh.seg_group[sg2];
           sh.ent_start[sg2] = e;
-          e += (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_rows[sg2] + 63) >> 6);
+          e += (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_hits[sg2] + 63) >> 6);
         }
         sh.ent_start[nseg] = e;
       }
+      // Whether any literal of the subset can match: one without a parent mask, or one whose
+      // parent matched some row.
+      for (int sg2 = 0; sg2 < nseg; ++sg2) {
+        const int g = uni(sh.seg_group[sg2]);
+        const int la = U->g[g].sbeg[s], nl = U->g[g].sbeg[s + 1] - la;
+        const int nb = (uni(sh.seg_hits[sg2]) + 63) >> 6;
+        for (int i = threadIdx.x; i < nl * nb; i += EV_THREADS) {
+          const int pm = lits[la + i / nb].pmask;
+          if (pm < 0 || masks[pm * a.mstride + i % nb] != 0) sh.need[s] = 1;
+        }
+      }
       if (hq < 0)
         for (uint32_t i = threadIdx.x, cap = table_cap_lds(np); i < cap; i += EV_THREADS) tbl[i] = 0;
-      __syncthreads();  // entry list and cleared table ready
-      if (uni(sh.ent_start[nseg]) == 0) continue;  // no literal of this subset in the item's groups
+      __syncthreads();  // entry list, need flag and cleared table ready
+      if (uni(sh.ent_start[nseg]) == 0 || uni(sh.need[s]) == 0) continue;
       const int nk = U->sub[s].nk;
       switch (nk) {
-        case 1: eval_subset<1>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
-        case 2: eval_subset<2>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
-        case 3: eval_subset<3>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
-        default: eval_subset<0>(a, U, sh, s, nk, hq, tbl, tstage, pstage, np, lane, wave); break;
+        case 1: eval_subset<1>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
+        case 2: eval_subset<2>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
+        case 3: eval_subset<3>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
+        default: eval_subset<0>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
       }
     }
   }
@@ -790,6 +931,8 @@ struct StepPlan {
   int64_t max_cells = 0, max_items = 0;
   size_t lds_bytes = 0;
   int max_blocks = 0;
+  int max_mask = 0;
+  int cidx_off = 0, mask_off = 0, mstride = 0;
   int num_slots = 0;
   int64_t bit_words = 0;
   uint64_t* pinned = nullptr;
@@ -808,6 +951,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   const bool cache = o.reuse_partitions != 0;
   P.num_slots = num_slots;
   P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
+  P.tile = (P.tile + 63) & ~63;  // whole 64-row blocks
   std::map<std::pair<int, int>, int> unit_R;
   for (const GroupSpec& g : specs) {
     const int R = o.radix_bits > 0 ? std::min(16, o.radix_bits)
@@ -892,23 +1036,34 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     du.ngroups = (int)S.groups.size();
     QF_REQUIRE(du.ngroups <= EV_MAXG, "too many join variables in one unit");
     du.arity = rp.pt->num_columns;
-    // Key subsets of the unit, in first-use order.
-    std::vector<std::vector<int>> subsets;
+    // Key subsets of the unit: subset 0 is the join column alone (every literal of the unit binds
+    // it); the others follow by key width, then first use.  Literals with row-side equalities
+    // get subsets of their own.
+    using SubKey = std::pair<std::vector<int>, bool>;
+    auto sub_key = [](const LitSpec* l) { return SubKey(l->pcols, !l->eqs.empty()); };
+    const int jc = S.key.second;
+    std::vector<SubKey> subsets{SubKey(std::vector<int>{jc}, false)};
     for (int gi = 0; gi < du.ngroups; ++gi)
       for (const LitSpec* l : S.lits[gi])
-        if (std::find(subsets.begin(), subsets.end(), l->pcols) == subsets.end()) subsets.push_back(l->pcols);
+        if (std::find(subsets.begin(), subsets.end(), sub_key(l)) == subsets.end()) subsets.push_back(sub_key(l));
+    std::stable_sort(subsets.begin() + 1, subsets.end(),
+                     [](const SubKey& x, const SubKey& y) { return x.first.size() < y.first.size(); });
     QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one unit");
-    int unit_lits = 0;
+    du.jcol = jc;
+    int unit_lits = 0, nmask = 0;
     for (int gi = 0; gi < du.ngroups; ++gi) {
       const int vi = var_index(S.groups[gi]->join_var);
       DevUGroup& G = du.g[gi];
       for (size_t c = 0; c <= P.bcols.size(); ++c) G.tcols[c] = P.tparts[vi][c].as<int32_t>();
       G.toff = P.toffs[vi].as<int64_t>();
+      G.jtcol = tidx(S.groups[gi]->join_var);
       // The group's literals ordered by subset; sbeg gives each subset's range.
+      std::vector<const LitSpec*> order;
+      const int first = (int)P.lits.size();
       for (int s = 0; s < (int)subsets.size(); ++s) {
         G.sbeg[s] = (int)P.lits.size();
         for (const LitSpec* l : S.lits[gi]) {
-          if (l->pcols != subsets[s]) continue;
+          if (sub_key(l) != subsets[s]) continue;
           DevLit dl{};
           dl.slot = l->slot;
           dl.subset = s;
@@ -919,19 +1074,47 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
             dl.eq[e][0] = tidx(l->eqs[e].first);
             dl.eq[e][1] = tidx(l->eqs[e].second);
           }
+          dl.pmask = -1;
+          dl.mslot = -1;
           P.lits.push_back(dl);
+          order.push_back(l);
           ++unit_lits;
         }
       }
       for (int s = (int)subsets.size(); s <= EV_MAXSUB; ++s) G.sbeg[s] = (int)P.lits.size();
+      // Parents: a literal of three or more key columns is pruned by a literal of the same group
+      // whose key is its key minus one column, bound to the same variables, with no row-side
+      // equalities (its matches contain the child's).  Two-column keys are already pruned by the
+      // join phase.
+      for (size_t i = 0; i < order.size(); ++i) {
+        const LitSpec* c = order[i];
+        if (c->pcols.size() < 3) continue;
+        for (size_t j = 0; j < order.size() && P.lits[first + i].pmask < 0; ++j) {
+          const LitSpec* par = order[j];
+          if (par->pcols.size() + 1 != c->pcols.size() || !par->eqs.empty()) continue;
+          bool contained = true;
+          for (size_t k = 0; k < par->pcols.size() && contained; ++k) {
+            const auto pos = std::find(c->pcols.begin(), c->pcols.end(), par->pcols[k]) - c->pcols.begin();
+            contained = pos < (ptrdiff_t)c->pcols.size() && c->bcols[pos] == par->bcols[k];
+          }
+          if (!contained) continue;
+          DevLit& pd = P.lits[first + j];
+          if (pd.mslot < 0) {
+            if (nmask >= EV_MAXMASK) break;
+            pd.mslot = nmask++;
+          }
+          P.lits[first + i].pmask = pd.mslot;
+        }
+      }
     }
     QF_REQUIRE(unit_lits <= EV_MAXLU, "too many literals in one unit");
+    P.max_mask = std::max(P.max_mask, nmask);
     du.nsub = (int)subsets.size();
     for (int s = 0; s < du.nsub; ++s) {
-      du.sub[s].nk = (int)subsets[s].size();
-      for (int k = 0; k < du.sub[s].nk; ++k) du.sub[s].pcol[k] = subsets[s][k];
+      du.sub[s].nk = (int)subsets[s].first.size();
+      for (int k = 0; k < du.sub[s].nk; ++k) du.sub[s].pcol[k] = subsets[s].first[k];
       if (!rp.heavy_q.empty()) {
-        auto ht = heavy_tables(ctx, rp, subsets[s]);
+        auto ht = heavy_tables(ctx, rp, subsets[s].first);
         du.sub[s].hbuf = ht->buf.as<uint32_t>();
         du.sub[s].hbase = ht->base.as<int64_t>();
         P.keep_heavy.push_back(ht);
@@ -971,7 +1154,12 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.d_items = DevBuf(ctx, std::max<int64_t>(1, P.max_items) * sizeof(DevItem));
   P.d_num_items = DevBuf(ctx, 8);
   P.d_counts = DevBuf(ctx, std::max(1, num_slots) * 4 * sizeof(uint64_t));
-  P.lds_bytes = (EV_TBL_WORDS + EV_PST_WORDS + (size_t)(P.bcols.size() + 1) * P.tile) * 4;
+  // LDS words: table | facts | staged rows ((ntc + 1) x tile) | hit-row list (uint16 x tile) |
+  // hit masks (uint64 x max_mask x tile / 64).
+  P.cidx_off = EV_TBL_WORDS + EV_PST_WORDS + (int)(P.bcols.size() + 1) * P.tile;
+  P.mask_off = P.cidx_off + P.tile / 2;
+  P.mstride = P.tile / 64;
+  P.lds_bytes = ((size_t)P.mask_off + (size_t)P.max_mask * P.mstride * 2) * 4;
   QF_REQUIRE(P.lds_bytes <= 160 * 1024 - 4096, "work item does not fit in LDS");
   QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)P.lds_bytes));
@@ -1000,6 +1188,9 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
     a.bits = reinterpret_cast<unsigned long long*>(d_bits);
     a.bit_words = P.bit_words;
     a.tile_rows = P.tile;
+    a.cidx_off = P.cidx_off;
+    a.mask_off = P.mask_off;
+    a.mstride = P.mstride;
     a.items = P.d_items.as<DevItem>();
     a.num_items = P.d_num_items.as<int64_t>();
     size_t uc = 0, li = 0;
