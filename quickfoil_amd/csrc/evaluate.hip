@@ -48,6 +48,7 @@ constexpr int EV_TBL_WORDS = 2048;     // LDS words for one count table (8 KB)
 constexpr int EV_PST_WORDS = 3072;     // LDS words for the partition's facts (12 KB)
 constexpr int EV_MIN_CAP = 16;
 constexpr int EV_MAXMASK = 64;         // literals per unit whose hit masks prune other literals
+constexpr int EV_MAXBU = 16;           // units per batch (units of equal radix bits share staged rows)
 
 struct DevSubset {
   int nk;
@@ -68,10 +69,8 @@ struct DevLit {
 };
 
 struct DevUGroup {
-  const int32_t* tcols[EV_MAXTC + 1];  // binding table partitioned on this join variable; [ntc] = row id
-  const int64_t* toff;
-  int jtcol;                           // staged column of the join variable
-  int sbeg[EV_MAXSUB + 1];             // literals of subset s: lits[sbeg[s], sbeg[s + 1]); subset 0 = {join column}
+  int jtcol;                // staged column of the join variable
+  int sbeg[EV_MAXSUB + 1];  // literals of subset s: lits[sbeg[s], sbeg[s + 1]); subset 0 = {join column}
 };
 
 struct DevUnit {
@@ -83,7 +82,8 @@ struct DevUnit {
   int arity;
   int ngroups;
   int nsub;
-  int jcol;  // the join column (subset 0's key)
+  int jcol;               // the join column (subset 0's key)
+  int ug_of_bg[EV_MAXG];  // the unit's group for each group of its batch, or -1
   DevUGroup g[EV_MAXG];
   DevSubset sub[EV_MAXSUB];
 };
@@ -224,8 +224,20 @@ __global__ __launch_bounds__(EV_THREADS) void k_heavy_finalize(DevRel rel, DevSu
 // ------------------------------------------------------------------------------------------
 // Work items.
 
+// Units with the same radix bits see the same binding cells; a batch of them stages each chunk
+// of binding rows once.  Its groups are the join variables of its units.
+struct DevBatch {
+  int R;
+  int shift;  // binding digits are `shift` bits finer
+  int ngroups;
+  int nunits;
+  const int32_t* tcols[EV_MAXG][EV_MAXTC + 1];  // binding table partitioned on the group's variable; [ntc] = row id
+  const int64_t* toff[EV_MAXG];
+  int unit[EV_MAXBU];
+};
+
 struct DevItem {
-  int unit;
+  int batch;
   int chunk;
   int64_t q;
 };
@@ -240,22 +252,22 @@ __device__ inline int find_unit(const int64_t* unit_cells, int num_units, int64_
   return lo;
 }
 
-__device__ inline int64_t unit_binding_rows(const DevUnit& U, int64_t q) {
-  int64_t rows = 0;
-  for (int g = 0; g < U.ngroups; ++g) rows += U.g[g].toff[(q + 1) << U.shift] - U.g[g].toff[q << U.shift];
-  return rows;
-}
-
-__global__ void k_count_items(const DevUnit* units, const int64_t* unit_cells, int num_units, int tile,
-                              int64_t* counts) {
+__global__ void k_count_items(const DevBatch* batches, const DevUnit* units, const int64_t* unit_cells,
+                              int num_units, int tile, int64_t* counts) {
   const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= unit_cells[num_units]) return;
-  const int u = find_unit(unit_cells, num_units, cell);
-  const DevUnit& U = units[u];
-  const int64_t q = cell - unit_cells[u];
-  const int64_t np = U.poff[q + 1] - U.poff[q];
-  const int64_t nt = np > 0 ? unit_binding_rows(U, q) : 0;
-  counts[cell] = (np > 0 && nt > 0) ? (nt + tile - 1) / tile : 0;
+  const int b = find_unit(unit_cells, num_units, cell);
+  const DevBatch& B = batches[b];
+  const int64_t q = cell - unit_cells[b];
+  bool facts = false;
+  for (int k = 0; k < B.nunits; ++k) {
+    const DevUnit& U = units[B.unit[k]];
+    facts = facts || U.poff[q + 1] > U.poff[q];
+  }
+  int64_t rows = 0;
+  if (facts)
+    for (int g = 0; g < B.ngroups; ++g) rows += B.toff[g][(q + 1) << B.shift] - B.toff[g][q << B.shift];
+  counts[cell] = (rows + tile - 1) / tile;
 }
 
 // Writes the items of every cell at the cell's scanned start; the last slot gets the total.
@@ -273,6 +285,7 @@ __global__ void k_expand_items(const int64_t* unit_cells, int num_units, const i
 }
 
 struct EvalArgs {
+  const DevBatch* batches;
   const DevUnit* units;
   const DevLit* lits;
   const DevItem* items;
@@ -290,7 +303,8 @@ struct EvalArgs {
 
 // Per-item shared state, filled by the block's first threads.
 struct ItemShared {
-  int seg_group[EV_MAXG];
+  int seg_group[EV_MAXG];      // batch group of the segment
+  int seg_ug[EV_MAXG];         // the current unit's group for the segment, or -1
   int seg_off[EV_MAXG + 1];
   int seg_rows[EV_MAXG];
   int64_t seg_src[EV_MAXG];
@@ -394,7 +408,7 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
   while (e0 >= uni(sh.blk_start[sg + 1])) ++sg;
   int blk = e0 - uni(sh.blk_start[sg]);
   int seg_rows = uni(sh.seg_rows[sg]), seg_off = uni(sh.seg_off[sg]), nblk = (seg_rows + 63) >> 6;
-  int g = uni(sh.seg_group[sg]);
+  int g = uni(sh.seg_ug[sg]);
   int jt = U->g[g].jtcol * tile, l0 = U->g[g].sbeg[0], l1 = U->g[g].sbeg[1];
   uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
   for (int e = e0; e < e1; ++e) {
@@ -429,11 +443,12 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
       flush_counts(a, lits, l0, l1, cp, cn, lbp, lbn, lane);
       blk = 0;
       if (e + 1 < e1) {
-        ++sg;
+        do ++sg;
+        while (e + 1 >= uni(sh.blk_start[sg + 1]));  // segments of groups the unit lacks have no blocks
         seg_rows = uni(sh.seg_rows[sg]);
         seg_off = uni(sh.seg_off[sg]);
         nblk = (seg_rows + 63) >> 6;
-        g = uni(sh.seg_group[sg]);
+        g = uni(sh.seg_ug[sg]);
         jt = U->g[g].jtcol * tile;
         l0 = U->g[g].sbeg[0];
         l1 = U->g[g].sbeg[1];
@@ -473,7 +488,7 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
   int nblk = (seg_hits + 63) >> 6;
   int lit = (e0 - uni(sh.ent_start[sg])) / nblk;
   int blk = (e0 - uni(sh.ent_start[sg])) - lit * nblk;
-  int lit0 = U->g[uni(sh.seg_group[sg])].sbeg[s];
+  int lit0 = U->g[uni(sh.seg_ug[sg])].sbeg[s];
   LitRegs L;
   load_literal<NK>(lits, lit0 + lit, nk, tile, L);
   uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
@@ -529,7 +544,7 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
           seg_off = uni(sh.seg_off[sg]);
           seg_end = uni(sh.ent_start[sg + 1]);
           nblk = (seg_hits + 63) >> 6;
-          lit0 = U->g[uni(sh.seg_group[sg])].sbeg[s];
+          lit0 = U->g[uni(sh.seg_ug[sg])].sbeg[s];
         }
         load_literal<NK>(lits, lit0 + lit, nk, tile, L);
       }
@@ -606,55 +621,66 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
   cptr<DevLit> lits = as_const(a.lits);
 
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
-    const int item_unit = items[it].unit, item_chunk = items[it].chunk;
+    const int item_batch = items[it].batch, item_chunk = items[it].chunk;
     const int64_t q = items[it].q;
-    cptr<DevUnit> U = as_const(a.units) + item_unit;
-    const int64_t p0 = U->poff[q];
-    const int np = (int)(U->poff[q + 1] - p0);
-    const int hq = U->hq[q];
+    cptr<DevBatch> B = as_const(a.batches) + item_batch;
 
     __syncthreads();  // the previous item is done with LDS
-    // Segments: the chunk's rows over the unit's join variables, in group order.
+    // Segments: the chunk's rows over the batch's join variables, in group order.
     if (threadIdx.x == 0) {
       const int64_t c0 = (int64_t)item_chunk * a.tile_rows, c1 = c0 + a.tile_rows;
       int64_t run = 0;
-      int ns = 0, off = 0, blocks = 0;
-      for (int g = 0; g < U->ngroups; ++g) {
-        const int64_t t0 = U->g[g].toff[q << U->shift], t1 = U->g[g].toff[(q + 1) << U->shift];
+      int ns = 0, off = 0;
+      for (int g = 0; g < B->ngroups; ++g) {
+        const int64_t t0 = B->toff[g][q << B->shift], t1 = B->toff[g][(q + 1) << B->shift];
         const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
         if (hi_ > lo_) {
-          const int n = (int)(hi_ - lo_);
           sh.seg_group[ns] = g;
           sh.seg_off[ns] = off;
-          sh.seg_rows[ns] = n;
+          sh.seg_rows[ns] = (int)(hi_ - lo_);
           sh.seg_src[ns] = t0 + (lo_ - run);
-          sh.seg_hits[ns] = 0;
-          sh.blk_start[ns] = blocks;
-          off += n;
-          blocks += (n + 63) >> 6;
+          off += (int)(hi_ - lo_);
           ++ns;
         }
         run += t1 - t0;
       }
       sh.seg_off[ns] = off;
-      sh.blk_start[ns] = blocks;
       sh.nseg = ns;
-      for (int s = 0; s < EV_MAXSUB; ++s) sh.need[s] = 0;
     }
     __syncthreads();
     const int nseg = uni(sh.nseg);
-    // Stage the binding rows of every segment (same staged columns in every group) and, for an
-    // LDS table, the partition's facts (all relation columns).
+    // Stage the binding rows of every segment (same staged columns in every group).
     for (int sgi = 0; sgi < nseg; ++sgi) {
       const int g = uni(sh.seg_group[sgi]);
       const int64_t src0 = uni64(sh.seg_src[sgi]);
       const int n = uni(sh.seg_rows[sgi]), dst0 = uni(sh.seg_off[sgi]);
       for (int c = 0; c <= ntc; ++c) {
-        const int32_t* src = U->g[g].tcols[c] + src0;
+        const int32_t* src = B->tcols[g][c] + src0;
         int32_t* dst = tstage + c * a.tile_rows + dst0;
         for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
       }
     }
+
+   for (int k = 0; k < B->nunits; ++k) {
+    cptr<DevUnit> U = as_const(a.units) + B->unit[k];
+    const int64_t p0 = U->poff[q];
+    const int np = (int)(U->poff[q + 1] - p0);
+    if (np == 0) continue;
+    const int hq = U->hq[q];
+    __syncthreads();  // staged rows ready; the previous unit is done with the table, facts, lists and masks
+    if (threadIdx.x == 0) {
+      int blocks = 0;
+      for (int sg2 = 0; sg2 < nseg; ++sg2) {
+        const int ug = U->ug_of_bg[sh.seg_group[sg2]];
+        sh.seg_ug[sg2] = ug;
+        sh.seg_hits[sg2] = 0;
+        sh.blk_start[sg2] = blocks;
+        if (ug >= 0) blocks += (sh.seg_rows[sg2] + 63) >> 6;
+      }
+      sh.blk_start[nseg] = blocks;
+      for (int s = 0; s < EV_MAXSUB; ++s) sh.need[s] = 0;
+    }
+    // The partition's facts (all relation columns) for an LDS table.
     if (hq < 0) {
       for (int c = 0; c < U->arity; ++c) {
         const int32_t* src = U->pcols[c] + p0;
@@ -662,7 +688,7 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
       }
       for (uint32_t i = threadIdx.x, cap = table_cap_lds(np); i < cap; i += EV_THREADS) tbl[i] = 0;
     }
-    __syncthreads();  // staged rows, facts and cleared table ready
+    __syncthreads();  // segment map, facts and cleared table ready
 
     // Join phase (subset 0: the join column alone).
     if (hq < 0) {
@@ -683,16 +709,17 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
       if (threadIdx.x == 0) {
         int e = 0;
         for (int sg2 = 0; sg2 < nseg; ++sg2) {
-          const int g = sh.seg_group[sg2];
+          const int g = sh.seg_ug[sg2];
           sh.ent_start[sg2] = e;
-          e += (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_hits[sg2] + 63) >> 6);
+          if (g >= 0) e += (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_hits[sg2] + 63) >> 6);
         }
         sh.ent_start[nseg] = e;
       }
       // Whether any literal of the subset can match: one without a parent mask, or one whose
       // parent matched some row.
       for (int sg2 = 0; sg2 < nseg; ++sg2) {
-        const int g = uni(sh.seg_group[sg2]);
+        const int g = uni(sh.seg_ug[sg2]);
+        if (g < 0) continue;
         const int la = U->g[g].sbeg[s], nl = U->g[g].sbeg[s + 1] - la;
         const int nb = (uni(sh.seg_hits[sg2]) + 63) >> 6;
         for (int i = threadIdx.x; i < nl * nb; i += EV_THREADS) {
@@ -712,6 +739,7 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
         default: eval_subset<0>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
       }
     }
+   }
   }
 }
 
@@ -923,11 +951,12 @@ struct StepPlan {
   std::vector<std::shared_ptr<RelPart>> keep_rel;
   std::vector<std::shared_ptr<HeavyTables>> keep_heavy;
   std::vector<DevUnit> units;
+  std::vector<DevBatch> batches;
   std::vector<DevLit> lits;
   std::vector<std::pair<int, int>> launches;  // unit ranges launched together
   std::vector<int64_t> unit_cells;            // per launch-local prefix, concatenated per launch
   std::vector<int64_t> launch_items;          // per launch: upper bound of its work items
-  DevBuf d_units, d_lits, d_unit_cells, d_cell_counts, d_cell_starts, d_items, d_num_items, d_counts;
+  DevBuf d_units, d_batches, d_lits, d_unit_cells, d_cell_counts, d_cell_starts, d_items, d_num_items, d_counts;
   int64_t max_cells = 0, max_items = 0;
   size_t lds_bytes = 0;
   int max_blocks = 0;
@@ -996,6 +1025,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     std::vector<std::vector<const LitSpec*>> lits;
   };
   std::vector<UnitSpec> us;
+  std::vector<std::vector<int>> unit_vars;  // per unit: the join variable of each of its groups
   std::map<std::pair<int, int>, int> uindex;
   for (const GroupSpec& g : specs) {
     const auto key = std::make_pair(g.relation, g.join_col);
@@ -1054,8 +1084,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     for (int gi = 0; gi < du.ngroups; ++gi) {
       const int vi = var_index(S.groups[gi]->join_var);
       DevUGroup& G = du.g[gi];
-      for (size_t c = 0; c <= P.bcols.size(); ++c) G.tcols[c] = P.tparts[vi][c].as<int32_t>();
-      G.toff = P.toffs[vi].as<int64_t>();
+      (void)vi;
       G.jtcol = tidx(S.groups[gi]->join_var);
       // The group's literals ordered by subset; sbeg gives each subset's range.
       std::vector<const LitSpec*> order;
@@ -1121,22 +1150,77 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       }
     }
     P.units.push_back(du);
+    std::vector<int> vars;
+    for (const GroupSpec* g : S.groups) vars.push_back(g->join_var);
+    unit_vars.push_back(vars);
   }
 
-  // Launch ranges: everything at once (batched) or unit by unit (plain forms).
-  const int U = (int)P.units.size();
+  // Batches: in the batched form, units with equal radix bits (they see the same binding cells,
+  // so each chunk of binding rows is staged once for all of them); otherwise one unit each.
+  std::vector<std::vector<int>> members;
   if (o.mode == QF_EVAL_BATCHED) {
-    if (U > 0) P.launches.emplace_back(0, U);
+    std::map<int, std::vector<int>> by_r;
+    for (int u = 0; u < (int)P.units.size(); ++u) by_r[P.units[u].R].push_back(u);
+    for (auto& kv : by_r) {
+      std::vector<int> cur;
+      std::set<int> cur_vars;
+      for (int u : kv.second) {
+        std::set<int> vs = cur_vars;
+        vs.insert(unit_vars[u].begin(), unit_vars[u].end());
+        if (!cur.empty() && ((int)cur.size() == EV_MAXBU || (int)vs.size() > EV_MAXG)) {
+          members.push_back(cur);
+          cur.clear();
+          vs = std::set<int>(unit_vars[u].begin(), unit_vars[u].end());
+        }
+        cur.push_back(u);
+        cur_vars = vs;
+      }
+      if (!cur.empty()) members.push_back(cur);
+    }
   } else {
-    for (int u = 0; u < U; ++u) P.launches.emplace_back(u, u + 1);
+    for (int u = 0; u < (int)P.units.size(); ++u) members.push_back({u});
+  }
+  for (const auto& m : members) {
+    DevBatch b{};
+    b.R = P.units[m[0]].R;
+    b.shift = P.R_T - b.R;
+    std::vector<int> bvars;
+    for (int u : m)
+      for (int v : unit_vars[u])
+        if (std::find(bvars.begin(), bvars.end(), v) == bvars.end()) bvars.push_back(v);
+    b.ngroups = (int)bvars.size();
+    for (int g = 0; g < b.ngroups; ++g) {
+      const int vi = var_index(bvars[g]);
+      for (size_t c = 0; c <= P.bcols.size(); ++c) b.tcols[g][c] = P.tparts[vi][c].as<int32_t>();
+      b.toff[g] = P.toffs[vi].as<int64_t>();
+    }
+    b.nunits = (int)m.size();
+    for (int k = 0; k < b.nunits; ++k) {
+      const int u = m[k];
+      b.unit[k] = u;
+      for (int g = 0; g < EV_MAXG; ++g) P.units[u].ug_of_bg[g] = -1;
+      for (int g = 0; g < b.ngroups; ++g) {
+        auto it = std::find(unit_vars[u].begin(), unit_vars[u].end(), bvars[g]);
+        if (it != unit_vars[u].end()) P.units[u].ug_of_bg[g] = (int)(it - unit_vars[u].begin());
+      }
+    }
+    P.batches.push_back(b);
+  }
+
+  // Launch ranges: every batch at once (batched) or one per launch (plain forms).
+  const int NB = (int)P.batches.size();
+  if (o.mode == QF_EVAL_BATCHED) {
+    if (NB > 0) P.launches.emplace_back(0, NB);
+  } else {
+    for (int b = 0; b < NB; ++b) P.launches.emplace_back(b, b + 1);
   }
   for (auto& lr : P.launches) {
     int64_t c = 0, items = 0;
-    for (int u = lr.first; u < lr.second; ++u) {
+    for (int b = lr.first; b < lr.second; ++b) {
       P.unit_cells.push_back(c);
-      c += (int64_t)1 << P.units[u].R;
-      // Items of a unit: at most one per cell plus one per tile of its binding rows.
-      items += ((int64_t)1 << P.units[u].R) + (int64_t)P.units[u].ngroups * ceil_div(binding.num_rows, P.tile);
+      c += (int64_t)1 << P.batches[b].R;
+      // Items of a batch: at most one per cell plus one per tile of its binding rows.
+      items += ((int64_t)1 << P.batches[b].R) + (int64_t)P.batches[b].ngroups * ceil_div(binding.num_rows, P.tile);
     }
     P.unit_cells.push_back(c);
     P.max_cells = std::max(P.max_cells, c);
@@ -1144,6 +1228,8 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     P.launch_items.push_back(items);
   }
   P.d_units = DevBuf(ctx, std::max<size_t>(1, P.units.size()) * sizeof(DevUnit));
+  P.d_batches = DevBuf(ctx, std::max<size_t>(1, P.batches.size()) * sizeof(DevBatch));
+  if (!P.batches.empty()) copy_h2d(ctx, P.d_batches.get(), P.batches.data(), P.batches.size() * sizeof(DevBatch));
   P.d_lits = DevBuf(ctx, std::max<size_t>(1, P.lits.size()) * sizeof(DevLit));
   P.d_unit_cells = DevBuf(ctx, std::max<size_t>(1, P.unit_cells.size()) * 8);
   if (!P.units.empty()) copy_h2d(ctx, P.d_units.get(), P.units.data(), P.units.size() * sizeof(DevUnit));
@@ -1198,14 +1284,15 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
       const int64_t items_bound = P.launch_items[li++];
       const int nu = lr.second - lr.first;
       const int64_t cells = P.unit_cells[uc + nu];
-      a.units = P.d_units.as<DevUnit>() + lr.first;
+      a.units = P.d_units.as<DevUnit>();  // batches name units by their index in the step
+      a.batches = P.d_batches.as<DevBatch>() + lr.first;
       const int64_t* unit_cells = P.d_unit_cells.as<int64_t>() + uc;
       uc += nu + 1;
       if (cells == 0) continue;
       {
         PhaseScope ps(ctx, "work_items");
         const dim3 grid((unsigned)ceil_div(cells, 256));
-        hipLaunchKernelGGL(k_count_items, grid, dim3(256), 0, ctx->stream, a.units, unit_cells, nu, P.tile,
+        hipLaunchKernelGGL(k_count_items, grid, dim3(256), 0, ctx->stream, a.batches, a.units, unit_cells, nu, P.tile,
                            P.d_cell_counts.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_cell_counts.as<int64_t>(), P.d_cell_starts.as<int64_t>(), cells);
