@@ -57,6 +57,18 @@ def algorithmic_probe_bytes(workload):
     return total, len(groups)
 
 
+def measured_traffic(workload):
+    """HBM bytes per k_eval launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    this workload (profiles/k_eval_traffic.json), or None when none was recorded."""
+    path = os.path.join(ROOT, "profiles", "k_eval_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    return rec.get("traffic_bytes_per_launch") if rec else None
+
+
 def cpu_baseline(workload, fraction, log_fn):
     """The oracle's restatement of the reference pipeline (1 thread) on the first `fraction`
     of the binding rows with every relation and literal."""
@@ -83,7 +95,7 @@ def main():
     ap.add_argument("--workload", choices=["step", "micro"], default="step")
     ap.add_argument("--dist", choices=["uniform", "zipf"], default="zipf")
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-fraction", type=float, default=1.0 / 64)
+    ap.add_argument("--cpu-fraction", type=float, default=1.0 / 256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
@@ -211,7 +223,7 @@ def main():
         "cold_step_ms": cold_s * 1e3,
         "phase_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in phases.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_eval (probe and count; HIP events around each launch on the context stream)",
                      "algorithmic_bytes_per_step": algo_bytes},
         "cpu_baseline": cpu,
