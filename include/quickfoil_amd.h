@@ -213,6 +213,17 @@ int qf_left_semi_join(qf_context* ctx, const qf_table* probe_keys, const qf_tabl
 int qf_right_semi_join(qf_context* ctx, const qf_table* probe_keys, const qf_table* build_keys,
                        uint64_t* d_out_bits, int64_t* h_num_ones);
 
+/* Example coverage of a literal: the semi-join chain of QuickFoil::ComputeCoverageOnUncoveredData
+ * and LiteralSelector::ComputeCoveredPositives (learner/QuickFoil.cpp:471-573,
+ * learner/LiteralSelector.cpp:150-210; SelectAndCreateSemiJoin + BuildHashTableAfterSemiJoin +
+ * CreateSemiJoin).  Binding rows [row_begin, row_end) that match `literal` (vars index binding
+ * columns; -1 = new) are projected to their first examples->num_columns columns; bit e of
+ * d_out_bits (ceil(examples rows / 64) words) = example row e equals one of those projections.
+ * *h_num_covered = number of set bits. */
+int qf_example_coverage(qf_context* ctx, const qf_table* binding, int64_t row_begin, int64_t row_end,
+                        const qf_table* relation, const qf_literal* literal, const qf_table* examples,
+                        uint64_t* d_out_bits, int64_t* h_num_covered);
+
 /* BuildHashTableAfterSemiJoin (operations/BuildHashTable.cpp:127-189,212-236): among rows set
  * in d_bits, keep the first row of each distinct key (InsertIfNotPresent order) and build the
  * chained table over them: d_buckets (P2(n)), d_next (n; entries of rows not inserted are 0),

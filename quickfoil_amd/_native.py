@@ -107,6 +107,8 @@ SIGNATURES = {
     "qf_left_semi_join": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_right_semi_join": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_build_hash_table_after_semi_join": (_I, [_P, ctypes.POINTER(QfTable), _P, _P, _P, _P]),
+    "qf_example_coverage": (_I, [_P, ctypes.POINTER(QfTable), _I64, _I64, ctypes.POINTER(QfTable),
+                                 ctypes.POINTER(QfLiteral), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_parse_relation_text": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _I64, ctypes.POINTER(_I64),
                                     ctypes.POINTER(_I64)]),
 }
@@ -400,6 +402,22 @@ class Context:
 
     def right_semi_join(self, probe_keys, build_keys):
         return self._semi(self.lib.qf_right_semi_join, probe_keys, build_keys, build_keys[0].numel())
+
+    def example_coverage(self, binding, row_begin, row_end, relation, vars_, examples):
+        """Bits over example rows covered by the binding rows [row_begin, row_end) that match the
+        literal (relation columns, variables) -- the learner's coverage semi-join chain."""
+        t = self.torch
+        b, r, e = _TableHolder(binding), _TableHolder(relation), _TableHolder(examples)
+        lit = QfLiteral()
+        lit.relation = 0
+        for i in range(MAX_ARITY):
+            lit.vars[i] = vars_[i] if i < len(vars_) else -1
+        ne = e.t.num_rows
+        bits = t.zeros(max(1, (ne + 63) // 64), dtype=t.int64, device=examples[0].device)
+        ones = _I64(0)
+        self._check(self.lib.qf_example_coverage(self.h, ctypes.byref(b.t), row_begin, row_end, ctypes.byref(r.t),
+                                                 ctypes.byref(lit), ctypes.byref(e.t), _ptr(bits), ctypes.byref(ones)))
+        return bits[: (ne + 63) // 64], ones.value
 
     def build_hash_table_after_semi_join(self, key_columns, bits):
         t = self.torch

@@ -437,3 +437,58 @@ extern "C" int qf_build_hash_table_after_semi_join(qf_context* c, const qf_table
     sync(ctx);
   });
 }
+
+// Example coverage of one literal (the semi-join chain of learner/QuickFoil.cpp:471-573 and
+// learner/LiteralSelector.cpp:150-210): binding rows [row_begin, row_end) that match the literal
+// on its bound arguments, projected to their first examples->num_columns columns (the head), then
+// bit e of d_out_bits = example row e equals one of those projections.
+extern "C" int qf_example_coverage(qf_context* c, const qf_table* binding, int64_t row_begin, int64_t row_end,
+                                   const qf_table* relation, const qf_literal* literal, const qf_table* examples,
+                        uint64_t* d_out_bits, int64_t* h_num_covered) {
+  return guarded(c, [&](Context* ctx) {
+    check_tab(binding, "binding");
+    check_tab(relation, "relation");
+    check_tab(examples, "examples");
+    QF_REQUIRE(literal && d_out_bits, "null argument");
+    QF_REQUIRE(row_begin >= 0 && row_begin <= row_end && row_end <= binding->num_rows, "row range out of bounds");
+    QF_REQUIRE(examples->num_columns <= binding->num_columns, "examples wider than the binding table");
+    std::vector<const int32_t*> bkeys, rkeys, head, ex;
+    for (int pos = 0; pos < relation->num_columns; ++pos) {
+      const int v = literal->vars[pos];
+      if (v < 0) continue;
+      QF_REQUIRE(v < binding->num_columns, "literal variable outside the binding table");
+      bkeys.push_back(binding->columns[v] + row_begin);
+      rkeys.push_back(relation->columns[pos]);
+    }
+    QF_REQUIRE(!bkeys.empty(), "literal has no bound variable");
+    for (int col = 0; col < examples->num_columns; ++col) {
+      head.push_back(binding->columns[col] + row_begin);
+      ex.push_back(examples->columns[col]);
+    }
+    const int64_t n = row_end - row_begin, ne = examples->num_rows;
+    DevBuf ones(ctx, 8);
+    fill_u64(ctx, ones.as<uint64_t>(), 0, 1);
+    if (ne > 0) fill_u64(ctx, d_out_bits, 0, ceil_div(ne, 64));
+    if (n > 0 && ne > 0 && relation->num_rows > 0) {
+      // Binding rows that match the literal.
+      DevBuf matched(ctx, (size_t)ceil_div(n, 64) * 8);
+      DevBuf scratch(ctx, 8);
+      const Keys kb = make_keys(bkeys), kr = make_keys(rkeys);
+      KeySet facts = build_keyset(ctx, kr, relation->num_rows, nullptr, nullptr);
+      hipLaunchKernelGGL(k_semi_bits, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, kb, n, kr,
+                         facts.state.as<uint32_t>(), facts.cap, matched.as<uint64_t>(),
+                         scratch.as<unsigned long long>());
+      QF_CHECK_HIP(hipGetLastError());
+      // Their head projections, probed by the examples.
+      const Keys kh = make_keys(head), ke = make_keys(ex);
+      KeySet heads = build_keyset(ctx, kh, n, matched.as<uint64_t>(), nullptr);
+      hipLaunchKernelGGL(k_semi_bits, dim3((unsigned)ceil_div(ne, 256)), dim3(256), 0, ctx->stream, ke, ne, kh,
+                         heads.state.as<uint32_t>(), heads.cap, d_out_bits, ones.as<unsigned long long>());
+      QF_CHECK_HIP(hipGetLastError());
+      sync(ctx);
+    }
+    int64_t h = 0;
+    copy_d2h(ctx, &h, ones.get(), 8);
+    if (h_num_covered) *h_num_covered = h;
+  });
+}

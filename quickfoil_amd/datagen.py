@@ -125,9 +125,9 @@ def kinship(num_persons=2000, generations=6, children_mean=2.5, seed=0x4B1E5, ne
         gen_of.append(g)
         return pid
 
-    # Founders sized so that the forest reaches num_persons by the last generation.
-    growth = sum((children_mean / 2.0 + 1.0) ** g for g in range(generations))
-    founders = max(4, int(num_persons / growth))
+    # Founders: an eighth of the persons; with Poisson(children_mean) children per couple and
+    # spouses joining each generation the forest reaches num_persons within the generations.
+    founders = max(4, num_persons // 8)
     current = [new_person(0) for _ in range(founders)]
     for g in range(generations):
         if next_id[0] >= num_persons:
@@ -209,3 +209,58 @@ def kinship(num_persons=2000, generations=6, children_mean=2.5, seed=0x4B1E5, ne
     }
     examples = cols(pos + neg)
     return {"relations": rel, "target": examples, "num_positive": len(pos), "num_persons": n}
+
+
+def bongard(num_pictures=200000, objects_mean=6.0, inside_density=0.15, noise=0.02, seed=0xB0A6D):
+    """Bongard-style synthetic pictures: element(picture, object) with Poisson(objects_mean)
+    objects per picture; circle/square/triangle and small/large on objects; inside(a, b) for 15 %
+    of same-picture object pairs.  The seeded rule `element(P, A), circle(A), small(A)` (length 3)
+    holds for half the pictures (planted, and kept false for the rest), then 2 % of the labels
+    flip.  Picture ids have type 0, object ids type 1.  Returns relations, the target column
+    (positives first) and the positive count."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = int(num_pictures)
+    counts = np.maximum(1, rng.poisson(objects_mean, n)).astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    m = int(starts[-1])
+    pic_of = np.repeat(np.arange(n, dtype=np.int64), counts)
+    shape = rng.integers(0, 3, m)          # 0 circle, 1 square, 2 triangle
+    size = rng.integers(0, 2, m)           # 0 small, 1 large
+    rule_true = rng.random(n) < 0.5
+    # Plant the rule in the first object of rule-true pictures; break it everywhere else.
+    first = starts[:-1]
+    shape[first[rule_true]] = 0
+    size[first[rule_true]] = 0
+    small_circle = (shape == 0) & (size == 0)
+    bad = small_circle & ~rule_true[pic_of]
+    size[bad] = 1
+    labels = rule_true ^ (rng.random(n) < noise)
+    obj = np.arange(m, dtype=np.int64) + n  # object ids follow picture ids
+    # inside: a fraction of the ordered same-picture pairs.
+    ins_a, ins_b = [], []
+    k = counts
+    pairs = k * (k - 1)
+    draws = rng.binomial(pairs, inside_density)
+    for p in np.nonzero(draws)[0]:
+        c = int(k[p])
+        idx = rng.choice(c * c, size=int(draws[p]) + c, replace=False)
+        a, b = idx // c, idx % c
+        keep = a != b
+        a, b = a[keep][: draws[p]], b[keep][: draws[p]]
+        ins_a.append(obj[starts[p] + a])
+        ins_b.append(obj[starts[p] + b])
+    ins_a = np.concatenate(ins_a) if ins_a else np.zeros(0, np.int64)
+    ins_b = np.concatenate(ins_b) if ins_b else np.zeros(0, np.int64)
+    i32 = lambda x: np.ascontiguousarray(x, dtype=np.int32)
+    rel = {
+        "element": ([i32(pic_of), i32(obj)], [0, 1]),
+        "circle": ([i32(obj[shape == 0])], [1]),
+        "square": ([i32(obj[shape == 1])], [1]),
+        "triangle": ([i32(obj[shape == 2])], [1]),
+        "small": ([i32(obj[size == 0])], [1]),
+        "large": ([i32(obj[size == 1])], [1]),
+        "inside": ([i32(ins_a), i32(ins_b)], [1, 1]),
+    }
+    pics = np.arange(n, dtype=np.int32)
+    target = np.concatenate([pics[labels], pics[~labels]])
+    return {"relations": rel, "target": [i32(target)], "num_positive": int(labels.sum())}

@@ -94,3 +94,26 @@ def test_hash_table_after_semi_join(qf_ctx):
             pos = int(nx[pos]) - 1
         want = [i for i in ins[::-1] if (oracle.hash_combine(oracle.hash_int(keys[0][i]), keys[1][i]) & (nb - 1)) == bucket]
         assert got == want
+
+
+@pytest.mark.parametrize("vars_,rows", [([0, -1], (0, 300)), ([1, 2], (100, 400)), ([2, 2], (0, 400)), ([-1, 0], (50, 60))])
+def test_example_coverage(qf_ctx, vars_, rows):
+    """Examples covered by the binding rows in range that match the literal (brute force)."""
+    rng = np.random.Generator(np.random.PCG64(sum(rows) + len(vars_)))
+    binding = [rng.integers(0, 12, 400).astype(np.int32) for _ in range(3)]
+    rel = [rng.integers(0, 12, 150).astype(np.int32) for _ in range(2)]
+    examples = [rng.integers(0, 12, 200).astype(np.int32) for _ in range(2)]
+    r0, r1 = rows
+    facts = {tuple(int(rel[p][s]) for p in range(2) if vars_[p] >= 0) for s in range(150)}
+    heads = set()
+    for r in range(r0, r1):
+        key = tuple(int(binding[v][r]) for v in vars_ if v >= 0)
+        if key in facts:
+            heads.add((int(binding[0][r]), int(binding[1][r])))
+    want = [(int(examples[0][e]), int(examples[1][e])) in heads for e in range(200)]
+    bits, ones = qf_ctx.example_coverage([_t(c) for c in binding], r0, r1, [_t(c) for c in rel], vars_,
+                                         [_t(c) for c in examples])
+    b = bits.cpu().numpy().view(np.uint64)
+    got = [bool((int(b[e >> 6]) >> (e & 63)) & 1) for e in range(200)]
+    assert got == want
+    assert ones == sum(want)

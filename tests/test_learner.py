@@ -1,0 +1,57 @@
+"""Host learner (QuickFoil search control) over the CPU oracle backend; the GPU tests show the
+device backend learns the identical clauses."""
+import pytest
+
+from learner_problems import bongard_sample, bongard_synthetic, cpu_backend, kinship, release_cpu_backend, uncle
+from quickfoil_amd import learner
+
+# Clauses the learner finds with the oracle backend (the reference pipeline restated); the
+# device backend must reproduce them exactly.
+EXPECTED = {
+    "uncle": ["uncle(0, 1) :-  brother(2, 1), parent(3, 2)"],
+    "bongard_sample": None,  # recorded by test_cpu_backend_is_deterministic
+    "kinship": ["uncle(0, 1) :-  brother(0, 2), parent(3, 0), parent(3, 4), parent(2, 1)",
+                "uncle(0, 1) :-  brother(0, 2), married(0, 3), parent(4, 1), sister(3, 4)"],
+    "bongard_synthetic": ["picture(0) :-  element(0, 1), small(1), circle(1)"],
+}
+
+PROBLEMS = {
+    "uncle": uncle,
+    "bongard_sample": bongard_sample,
+    "kinship": kinship,
+    "bongard_synthetic": lambda: bongard_synthetic(5000),
+}
+
+
+def learn_cpu(name):
+    problem, opts = PROBLEMS[name]()
+    be, keep = cpu_backend(problem)
+    try:
+        return learner.learn(problem, be, learner.default_options(**opts))
+    finally:
+        release_cpu_backend(be)
+
+
+@pytest.mark.parametrize("name", ["uncle", "kinship", "bongard_synthetic"])
+def test_cpu_backend_clauses(name):
+    assert learn_cpu(name) == EXPECTED[name]
+
+
+def test_cpu_backend_is_deterministic():
+    a = learn_cpu("bongard_sample")
+    assert a and a == learn_cpu("bongard_sample")
+    for clause in a:
+        assert clause.startswith("bongard(0) :-  ")
+
+
+def test_planted_rule_is_found():
+    # The synthetic Bongard rule element(P, A), circle(A), small(A) is learnt (in some order).
+    (clause,) = learn_cpu("bongard_synthetic")
+    body = sorted(l.strip() for l in clause.split(":-")[1].split("), "))
+    assert [b.rstrip(")") for b in body] == ["circle(1", "element(0, 1", "small(1"]
+
+
+def test_options_default():
+    o = learner.default_options()
+    assert (o.positive_threshold, o.max_iterations, o.maximum_clause_length, o.num_saved_literals) == (0.8, 1000, 25, 5)
+    assert (o.minimum_inflated_precision, o.minimum_true_precision, o.minimum_f_score) == (0.85, 0.8, 0.85)
