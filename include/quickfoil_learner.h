@@ -40,6 +40,10 @@ typedef struct qf_learner_backend {
   int (*coverage)(void* self, void* binding, int num_columns, int64_t row_begin, int64_t row_end,
                   const qf_literal* literal, void* examples, int num_example_columns, int64_t num_examples,
                   uint64_t* h_bits, int64_t* num_covered);
+  /* CreateBindingTable (operations/MultiColumnHashJoin.cpp:26-115): the binding table joined
+   * with a literal whose variables >= num_columns are new columns; *out_rows = its rows. */
+  int (*join)(void* self, void* binding, int num_columns, int64_t num_rows, const qf_literal* literal, void** out,
+              int64_t* out_rows);
 } qf_learner_backend;
 
 /* Learner flags and their reference defaults (learner/QuickFoil.cpp:38-73,
@@ -78,6 +82,13 @@ void qf_learner_options_default(qf_learner_options* options);
  * *num_clauses receives their number.  QF_ERR_OVERFLOW when `out` is too small. */
 int qf_learn(const qf_learner_problem* problem, const qf_learner_options* options,
              const qf_learner_backend* backend, char* out, int64_t out_capacity, int* num_clauses);
+
+/* QuickFoilTestRunner::RunTest (learner/QuickFoilTestRunner.cpp:135-160): test examples (head
+ * arity columns) left uncovered by the clauses.  Clause c has body_lengths[c] body literals,
+ * concatenated in `body` (variables index the clause's variables; the head binds 0..arity-1). */
+int qf_learner_test(const qf_learner_backend* backend, const int32_t* const* test_columns, int head_arity,
+                    int64_t num_tests, const qf_literal* body, const int* body_lengths, int num_clauses,
+                    int64_t* num_uncovered);
 
 /* Backend over libquickfoil_amd on a context: relations are the host columns given here (uploaded
  * once); tables live in device memory.  Release with qf_learner_device_backend_destroy. */

@@ -147,6 +147,28 @@ int coverage(void* self, void* binding, int ncols, int64_t r0, int64_t r1, const
   return QF_OK;
 }
 
+int join(void* self, void* binding, int ncols, int64_t rows, const qf_literal* lit, void** out, int64_t* out_rows) {
+  CpuBackend* b = static_cast<CpuBackend*>(self);
+  std::vector<const int32_t*> bc = ptrs(*static_cast<HostTable*>(binding), ncols);
+  std::vector<int> lr{lit->relation}, lv(lit->vars, lit->vars + ORC_MAX_ARITY);
+  orc_eval_input in = input(b, bc, rows, 0, lr, lv);
+  const int64_t n = orc_create_binding_table(&in, 0, nullptr);
+  if (n < 0) return QF_ERR_INVALID;
+  int extra = 0;
+  for (int i = 0; i < b->arity[lit->relation]; ++i) extra += lit->vars[i] >= ncols ? 1 : 0;
+  HostTable* t = new HostTable;
+  t->rows = n;
+  std::vector<int32_t*> oc;
+  for (int c = 0; c < ncols + extra; ++c) {
+    t->cols.emplace_back((size_t)(n + 1), 0);
+    oc.push_back(t->cols.back().data());
+  }
+  orc_create_binding_table(&in, 0, oc.data());
+  *out = t;
+  *out_rows = n;
+  return QF_OK;
+}
+
 }  // namespace
 
 extern "C" int orc_learner_backend_create(int num_relations, const int* arities,
@@ -177,6 +199,7 @@ extern "C" int orc_learner_backend_create(int num_relations, const int* arities,
   backend->evaluate = evaluate;
   backend->extend = extend;
   backend->coverage = coverage;
+  backend->join = join;
   return QF_OK;
 }
 

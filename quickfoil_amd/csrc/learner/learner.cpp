@@ -433,3 +433,74 @@ int qf_learn(const qf_learner_problem* problem, const qf_learner_options* option
 }
 
 }  // extern "C"
+
+// QuickFoilTestRunner (learner/QuickFoilTestRunner.cpp:40-161): per clause, the test examples are
+// joined with every body literal but the last (CreateBindingTable); the examples covered through
+// the last literal (the coverage semi-join chain, ComputeUnCoveredData :44-133) leave the
+// uncovered set, which keeps its row order.
+extern "C" int qf_learner_test(const qf_learner_backend* be, const int32_t* const* test_columns, int h,
+                               int64_t num_tests, const qf_literal* body, const int* body_lengths, int num_clauses,
+                               int64_t* num_uncovered) {
+  if (!be || !body_lengths || h < 1 || num_tests < 0 || (num_tests > 0 && !test_columns)) return QF_ERR_INVALID;
+  using qfl::check;
+  using qfl::Table;
+  using TablePtr = std::shared_ptr<Table>;
+  try {
+    auto upload = [&](const std::vector<const int32_t*>& cols, int64_t rows) {
+      void* t = nullptr;
+      check(be->upload(be->self, cols.data(), h, rows, &t), "upload");
+      return TablePtr(new Table(be, t, h, rows));
+    };
+    std::vector<const int32_t*> tc(test_columns, test_columns + h);
+    TablePtr tests = upload(tc, num_tests);
+    TablePtr uncovered = tests;
+    int64_t off = 0;
+    for (int c = 0; c < num_clauses; ++c) {
+      const int nb = body_lengths[c];
+      if (nb <= 0) return QF_ERR_INVALID;
+      const qf_literal* lits = body + off;
+      off += nb;
+      TablePtr binding = tests;
+      int ncols = h;
+      bool empty = num_tests == 0;
+      for (int i = 0; i + 1 < nb && !empty; ++i) {
+        void* out = nullptr;
+        int64_t rows = 0;
+        check(be->join(be->self, binding->h, ncols, binding->rows, &lits[i], &out, &rows), "join");
+        int extra = 0;
+        for (int k = 0; k < QF_MAX_ARITY; ++k) extra += lits[i].vars[k] >= ncols ? 1 : 0;
+        ncols += extra;
+        binding = TablePtr(new Table(be, out, ncols, rows));
+        empty = rows == 0;
+      }
+      if (empty) continue;  // the clause covers no test example
+      qf_literal last = lits[nb - 1];
+      for (int k = 0; k < QF_MAX_ARITY; ++k)
+        if (last.vars[k] >= ncols) last.vars[k] = -1;
+      std::vector<uint64_t> bits((size_t)std::max<int64_t>(1, (uncovered->rows + 63) / 64), 0);
+      int64_t covered = 0;
+      check(be->coverage(be->self, binding->h, ncols, 0, binding->rows, &last, uncovered->h, h, uncovered->rows,
+                         bits.data(), &covered),
+            "coverage");
+      const int64_t n = uncovered->rows;
+      std::vector<std::vector<int32_t>> cols(h, std::vector<int32_t>((size_t)std::max<int64_t>(1, n)));
+      std::vector<int32_t*> ptrs;
+      for (auto& col : cols) ptrs.push_back(col.data());
+      if (n > 0) check(be->download(be->self, uncovered->h, h, n, ptrs.data()), "download");
+      std::vector<std::vector<int32_t>> kept(h);
+      for (int64_t r = 0; r < n; ++r)
+        if (!((bits[r >> 6] >> (r & 63)) & 1ull))
+          for (int k = 0; k < h; ++k) kept[k].push_back(cols[k][r]);
+      std::vector<const int32_t*> kp;
+      for (auto& k : kept) kp.push_back(k.data());
+      uncovered = upload(kp, (int64_t)kept[0].size());
+      if (uncovered->rows == 0) break;
+    }
+    if (num_uncovered) *num_uncovered = uncovered->rows;
+    return QF_OK;
+  } catch (const qfl::LearnError& e) {
+    return e.status;
+  } catch (const std::exception&) {
+    return QF_ERR_INVALID;
+  }
+}

@@ -156,6 +156,32 @@ int be_coverage(void* self, void* binding, int ncols, int64_t r0, int64_t r1, co
   return st;
 }
 
+int be_join(void* self, void* binding, int ncols, int64_t rows, const qf_literal* lit, void** out, int64_t* out_rows) {
+  DeviceBackend* b = static_cast<DeviceBackend*>(self);
+  if (lit->relation < 0 || lit->relation >= (int)b->rel_tables.size()) return QF_ERR_INVALID;
+  const qf_table& rel = b->rel_tables[lit->relation];
+  int extra = 0;
+  for (int i = 0; i < rel.num_columns; ++i) extra += lit->vars[i] >= ncols ? 1 : 0;
+  std::vector<const int32_t*> p;
+  qf_table bt = view(static_cast<DevTable*>(binding), &p, ncols);
+  bt.num_rows = rows;
+  int64_t n = 0;
+  int st = qf_create_binding_table(b->ctx, &bt, &rel, lit, nullptr, &n);
+  if (st != QF_OK) return st;
+  DevTable* o = nullptr;
+  st = alloc_table(ncols + extra, n, &o);
+  if (st != QF_OK) return st;
+  st = qf_create_binding_table(b->ctx, &bt, &rel, lit, o->cols.data(), &n);
+  if (st == QF_OK) st = qf_synchronize(b->ctx);
+  if (st != QF_OK) {
+    be_release(self, o);
+    return st;
+  }
+  *out = o;
+  *out_rows = n;
+  return QF_OK;
+}
+
 }  // namespace
 
 extern "C" int qf_learner_device_backend_create(qf_context* ctx, int num_relations, const int* arities,
@@ -197,6 +223,7 @@ extern "C" int qf_learner_device_backend_create(qf_context* ctx, int num_relatio
   backend->evaluate = be_evaluate;
   backend->extend = be_extend;
   backend->coverage = be_coverage;
+  backend->join = be_join;
   return QF_OK;
 }
 

@@ -22,7 +22,7 @@ _I64 = ctypes.c_int64
 
 class LearnerBackend(ctypes.Structure):
     _fields_ = [("self", _P), ("upload", _P), ("download", _P), ("release", _P), ("concat", _P),
-                ("evaluate", _P), ("extend", _P), ("coverage", _P)]
+                ("evaluate", _P), ("extend", _P), ("coverage", _P), ("join", _P)]
 
 
 class LearnerOptions(ctypes.Structure):
@@ -64,6 +64,9 @@ def _bind(lib):
     lib.qf_learner_device_backend_create.argtypes = [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.POINTER(_P)),
                                                      ctypes.POINTER(_I64), ctypes.POINTER(ctypes.POINTER(_I)),
                                                      ctypes.POINTER(LearnerBackend)]
+    lib.qf_learner_test.restype = _I
+    lib.qf_learner_test.argtypes = [ctypes.POINTER(LearnerBackend), ctypes.POINTER(_P), _I, _I64,
+                                    ctypes.POINTER(_native.QfLiteral), ctypes.POINTER(_I), _I, ctypes.POINTER(_I64)]
     lib.qf_learner_device_backend_destroy.restype = None
     lib.qf_learner_device_backend_destroy.argtypes = [ctypes.POINTER(LearnerBackend)]
     return lib
@@ -199,3 +202,51 @@ def load_configuration(path, data_dir=None):
     if test is not None:
         test_setting = (os.path.join(base, os.path.basename(test["file"])), test["num_positive"])
     return Problem(target[0], target[1], background), test_setting
+
+
+def parse_clause(text, problem):
+    """FoilClause::ToString text -> list of body literals [(relation index, [variable ids])]."""
+    index = {r.name: i for i, r in enumerate(problem.background)}
+    body = text.split(":-", 1)[1].strip()
+    lits = []
+    for m in re.finditer(r"([^\s(),]+)\(([^)]*)\)", body):
+        lits.append((index[m.group(1)], [int(v) for v in m.group(2).split(",")]))
+    return lits
+
+
+def run_test(problem, backend, clauses, test_columns):
+    """QuickFoilTestRunner::RunTest (learner/QuickFoilTestRunner.cpp:135-160): the number of test
+    examples no clause covers."""
+    lib = _bind(_native.load_library())
+    cols = [np.ascontiguousarray(c, dtype=np.int32) for c in test_columns]
+    body, lengths = [], []
+    for c in clauses:
+        lits = parse_clause(c, problem)
+        lengths.append(len(lits))
+        for rel, vars_ in lits:
+            q = _native.QfLiteral()
+            q.relation = rel
+            for i in range(_native.MAX_ARITY):
+                q.vars[i] = vars_[i] if i < len(vars_) else -1
+            body.append(q)
+    arr = (_native.QfLiteral * max(1, len(body)))(*body)
+    lens = (_I * max(1, len(lengths)))(*lengths)
+    ptrs = (_P * len(cols))(*[c.ctypes.data for c in cols])
+    out = _I64(0)
+    st = lib.qf_learner_test(ctypes.byref(backend), ptrs, len(cols), int(cols[0].size), arr, lens, len(lengths),
+                             ctypes.byref(out))
+    if st != 0:
+        raise _native.QfError(st, "test run")
+    return out.value
+
+
+def test_metrics(problem, backend, clauses, test_columns, num_test_positive):
+    """Covered test positives/negatives, precision and recall as main/main.cpp:264-290 reports."""
+    pos = [c[:num_test_positive] for c in test_columns]
+    neg = [c[num_test_positive:] for c in test_columns]
+    cov_pos = num_test_positive - run_test(problem, backend, clauses, pos)
+    n_neg = int(neg[0].size)
+    cov_neg = n_neg - run_test(problem, backend, clauses, neg) if n_neg else 0
+    precision = 0.0 if cov_pos == 0 and cov_neg == 0 else cov_pos / (cov_pos + cov_neg)
+    return {"covered_test_positive": cov_pos, "covered_test_negative": cov_neg, "precision": precision,
+            "recall": cov_pos / num_test_positive}

@@ -55,3 +55,44 @@ def test_options_default():
     o = learner.default_options()
     assert (o.positive_threshold, o.max_iterations, o.maximum_clause_length, o.num_saved_literals) == (0.8, 1000, 25, 5)
     assert (o.minimum_inflated_precision, o.minimum_true_precision, o.minimum_f_score) == (0.85, 0.8, 0.85)
+
+
+def test_test_runner_on_bongard_sample():
+    """QuickFoilTestRunner over the sample's test file (10 positive tests, bongard.json:17-22)."""
+    import os
+    from learner_problems import GOLDEN
+    from quickfoil_amd.loader import load_relation
+    problem, opts = bongard_sample()
+    tests = load_relation(os.path.join(GOLDEN, "bongard100", "bongard_test"), 1)
+    be, keep = cpu_backend(problem)
+    try:
+        clauses = learner.learn(problem, be, learner.default_options(**opts))
+        m = learner.test_metrics(problem, be, clauses, tests, 10)
+        # Consistency: uncovered counts by brute force over the clause bodies are not available
+        # without a join engine, so check the bounds and the single-clause monotonicity.
+        assert 0 <= m["covered_test_positive"] <= 10 and 0 <= m["covered_test_negative"] <= 10
+        first = learner.test_metrics(problem, be, clauses[:1], tests, 10)
+        assert first["covered_test_positive"] <= m["covered_test_positive"]
+        assert first["covered_test_negative"] <= m["covered_test_negative"]
+    finally:
+        release_cpu_backend(be)
+
+
+def test_test_runner_planted_rule():
+    """On fresh synthetic pictures the planted rule covers exactly the rule-true ones."""
+    import numpy as np
+    from quickfoil_amd import datagen
+    problem, opts = bongard_synthetic(3000)
+    be, keep = cpu_backend(problem)
+    try:
+        clauses = ["picture(0) :-  element(0, 1), small(1), circle(1)"]
+        rel = {r.name: r for r in problem.background}
+        pics = np.unique(rel["element"].columns[0])
+        obj_pic = dict(zip(rel["element"].columns[1].tolist(), rel["element"].columns[0].tolist()))
+        sc = set(rel["small"].columns[0].tolist()) & set(rel["circle"].columns[0].tolist())
+        true_pics = {obj_pic[o] for o in sc}
+        tests = [np.array(sorted(true_pics) + sorted(set(pics.tolist()) - true_pics), dtype=np.int32)]
+        m = learner.test_metrics(problem, be, clauses, tests, len(true_pics))
+        assert m["covered_test_positive"] == len(true_pics) and m["covered_test_negative"] == 0
+    finally:
+        release_cpu_backend(be)
