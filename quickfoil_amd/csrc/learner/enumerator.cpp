@@ -65,7 +65,8 @@ void Enumerator::generate_and_prune(bool has_key, const Clause& clause, const Li
   if (bit != pred_to_body.end()) body_with_pred = &bit->second;
 
   VariableSet key_vars;
-  if (has_key && body_with_pred)
+  // A key outside the argument list selects no variable.
+  if (has_key && body_with_pred && pred->key < pred->arity())
     for (const Literal* bl : *body_with_pred) key_vars.insert(bl->var(pred->key));
 
   std::vector<std::vector<Variable>> per_arg;

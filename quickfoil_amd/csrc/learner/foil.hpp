@@ -147,6 +147,21 @@ class Clause {
     body_.emplace_back(l.predicate(), lv);
     random_.push_back(is_random);
   }
+  // FoilClause::AddBoundBodyLiteral (schema/FoilClause.cpp:63-79), used when a clause is read
+  // back from text: variables keep their ids.
+  void add_bound_body_literal(const Literal& l, bool is_random) {
+    nv_wo_last_ = (int)vars_.size();
+    for (const Variable& v : l.vars()) {
+      if (v.id >= (int)vars_.size()) {
+        vars_.resize(v.id);
+        vars_.push_back(v);
+      } else if (!vars_[v.id].bound()) {
+        vars_[v.id] = v;
+      }
+    }
+    body_.push_back(l);
+    random_.push_back(is_random);
+  }
   Literal unbound_last_literal() const { return body_.back().unbound_from(nv_wo_last_); }
   // FoilClause::ToString (schema/FoilClause.cpp:31-41).
   std::string to_string() const {
