@@ -202,6 +202,15 @@ int qf_create_binding_table(qf_context* ctx, const qf_table* binding, const qf_t
                             const qf_literal* literal, int32_t* const* d_out_columns,
                             int64_t* h_num_rows);
 
+/* CreateBindingTable into caller buffers of `capacity` rows sized from an estimate (the
+ * learned cardinality estimator presizes them): one pass when the output fits; otherwise
+ * QF_ERR_OVERFLOW with the exact row count in *h_num_rows and nothing written, and the caller
+ * retries with that size (exact fallback).  No counterpart in the reference, which has only the
+ * growing-buffer form (operations/MultiColumnHashJoin.cpp:445-461). */
+int qf_create_binding_table_presized(qf_context* ctx, const qf_table* binding, const qf_table* relation,
+                                     const qf_literal* literal, int64_t capacity, int32_t* const* d_out_columns,
+                                     int64_t* h_num_rows);
+
 /* LeftSemiJoin::Next over all chunks (operations/LeftSemiJoin.hpp:56-141): bit i of
  * d_out_bits (ceil(probe rows/64) words) = probe row i has a build row with equal keys.
  * probe_keys and build_keys hold the key columns (same count, 1..QF_MAX_ARITY). */
@@ -246,6 +255,28 @@ int qf_build_hash_table_after_semi_join(qf_context* ctx, const qf_table* keys,
 int qf_parse_relation_text(const char* text, int64_t length, int num_fields, const int* keep,
                            int32_t* const* h_columns, int64_t capacity, int64_t* h_num_rows,
                            int64_t* h_error_line);
+
+/* ---- Learned join-cardinality estimator (no counterpart in the reference) ----
+ * A 40 -> 64 -> 64 -> 1 ReLU MLP, BF16 operands with FP32 accumulation on MFMA, predicting
+ * log2(1 + output rows) of a join from 40 host-computed features (inputs row-major, float32).
+ * Its only use is presizing (qf_create_binding_table_presized), so no result depends on it.
+ * Parameters: qf_mlp_num_params() floats, W1[64][48] (columns 40..47 unused), b1[64],
+ * W2[64][64], b2[64], w3[64], b3.  The model's device memory comes from the context, which must
+ * outlive it. */
+typedef struct qf_mlp qf_mlp;
+int qf_mlp_num_params(void);
+int qf_mlp_create(qf_context* ctx, const float* h_params, qf_mlp** out);
+void qf_mlp_destroy(qf_mlp* mlp);
+int qf_mlp_get_params(qf_context* ctx, qf_mlp* mlp, float* h_params);
+/* Batched inference: d_out[i] = prediction for row i of d_x (n x 40). */
+int qf_mlp_forward(qf_context* ctx, qf_mlp* mlp, const float* d_x, int64_t n, float* d_out);
+/* Gradients of mean((pred - y)^2) over the n rows (h_grads: qf_mlp_num_params() floats, may be
+ * NULL) and the loss. */
+int qf_mlp_gradients(qf_context* ctx, qf_mlp* mlp, const float* d_x, const float* d_y, int64_t n, float* h_grads,
+                     float* h_loss);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) on the gradients of the n rows. */
+int qf_mlp_train_step(qf_context* ctx, qf_mlp* mlp, const float* d_x, const float* d_y, int64_t n, float lr,
+                      float* h_loss);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,15 @@ SIGNATURES = {
                                  ctypes.POINTER(QfLiteral), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_parse_relation_text": (_I, [ctypes.c_char_p, _I64, _I, _P, _P, _I64, ctypes.POINTER(_I64),
                                     ctypes.POINTER(_I64)]),
+    "qf_create_binding_table_presized": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable),
+                                              ctypes.POINTER(QfLiteral), _I64, _P, ctypes.POINTER(_I64)]),
+    "qf_mlp_num_params": (_I, []),
+    "qf_mlp_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
+    "qf_mlp_destroy": (None, [_P]),
+    "qf_mlp_get_params": (_I, [_P, _P, _P]),
+    "qf_mlp_forward": (_I, [_P, _P, _P, _I64, _P]),
+    "qf_mlp_gradients": (_I, [_P, _P, _P, _P, _I64, _P, ctypes.POINTER(ctypes.c_float)]),
+    "qf_mlp_train_step": (_I, [_P, _P, _P, _P, _I64, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None
@@ -386,6 +395,30 @@ class Context:
         self._check(self.lib.qf_create_binding_table(self.h, ctypes.byref(b.t), ctypes.byref(r.t), ctypes.byref(lit),
                                                      arr, ctypes.byref(rows)))
         return [o[: rows.value] for o in outs]
+
+    def create_binding_table_presized(self, binding_columns, relation_columns, vars_, capacity):
+        """CreateBindingTable into buffers of `capacity` rows; on overflow the exact size is used
+        for a second call.  Returns (columns, fell_back)."""
+        t = self.torch
+        b = _TableHolder(binding_columns)
+        r = _TableHolder(relation_columns)
+        lit = self._literal(0, vars_)
+        n_new = sum(1 for v in vars_ if v < 0 or v >= len(binding_columns))
+        dev = binding_columns[0].device
+        outs = [t.empty(max(1, capacity), dtype=t.int32, device=dev) for _ in range(len(binding_columns) + n_new)]
+        arr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        rows = ctypes.c_int64(0)
+        st = self.lib.qf_create_binding_table_presized(self.h, ctypes.byref(b.t), ctypes.byref(r.t),
+                                                       ctypes.byref(lit), capacity, arr, ctypes.byref(rows))
+        if st == 0:
+            return [o[: rows.value] for o in outs], False
+        if st != 4:  # anything but the overflow that asks for the exact size
+            self._check(st)
+        outs = [t.empty(max(1, rows.value), dtype=t.int32, device=dev) for _ in range(len(binding_columns) + n_new)]
+        arr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        self._check(self.lib.qf_create_binding_table(self.h, ctypes.byref(b.t), ctypes.byref(r.t), ctypes.byref(lit),
+                                                     arr, ctypes.byref(rows)))
+        return [o[: rows.value] for o in outs], True
 
     def _semi(self, fn, probe_keys, build_keys, nbits):
         t = self.torch

@@ -159,7 +159,7 @@ static KeySet build_keyset(Context* ctx, const Keys& k, int64_t n, const uint64_
 // any count).
 static int64_t ordered_join(Context* ctx, const std::vector<const int32_t*>& outer_keys, int64_t n_outer,
                             const std::vector<const int32_t*>& inner_keys, int64_t n_inner,
-                            const ProjectArgs& proj, bool write, int64_t expect) {
+                            const ProjectArgs& proj, bool write, int64_t expect, int64_t capacity = -1) {
   if (n_outer == 0 || n_inner == 0) return 0;
   const Keys ko = make_keys(outer_keys), ki = make_keys(inner_keys);
   DevBuf slot_of(ctx, n_inner * 4);
@@ -181,7 +181,7 @@ static int64_t ordered_join(Context* ctx, const std::vector<const int32_t*>& out
   copy_d2h(ctx, &last_off, out_off.as<int64_t>() + n_outer - 1, 8);
   copy_d2h(ctx, &last_cnt, counts.as<int64_t>() + n_outer - 1, 8);
   const int64_t total = last_off + last_cnt;
-  if (!write || total == 0 || (expect >= 0 && total != expect)) return total;
+  if (!write || total == 0 || (expect >= 0 && total != expect) || (capacity >= 0 && total > capacity)) return total;
   hipLaunchKernelGGL(k_project, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, ctx->stream, proj,
                      out_off.as<int64_t>(), seg_end.as<int64_t>(), sorted_row.as<int32_t>(), n_outer, total);
   QF_CHECK_HIP(hipGetLastError());
@@ -276,8 +276,9 @@ int qf_create_label_aware_binding_table(qf_context* c, const qf_table* binding, 
   });
 }
 
-int qf_create_binding_table(qf_context* c, const qf_table* binding, const qf_table* relation,
-                            const qf_literal* literal, int32_t* const* d_out_columns, int64_t* h_num_rows) {
+static int create_binding_table(qf_context* c, const qf_table* binding, const qf_table* relation,
+                                const qf_literal* literal, int32_t* const* d_out_columns, int64_t* h_num_rows,
+                                int64_t capacity) {
   return guarded(c, [&](Context* ctx) {
     check_tab(binding, "binding table");
     check_tab(relation, "relation");
@@ -311,13 +312,26 @@ int qf_create_binding_table(qf_context* c, const qf_table* binding, const qf_tab
       else p.inner_cols[col] = src;
     }
     const bool write = d_out_columns != nullptr;
-    const int64_t expect = write ? *h_num_rows : -1;
+    const int64_t expect = write && capacity < 0 ? *h_num_rows : -1;
     int64_t total;
-    if (binding_outer) total = ordered_join(ctx, bind_keys, N, lit_keys, nbg, p, write, expect);
-    else total = ordered_join(ctx, lit_keys, nbg, bind_keys, N, p, write, expect);
-    if (write && total != expect) throw Error(QF_ERR_OVERFLOW, "output size differs from the row count");
+    if (binding_outer) total = ordered_join(ctx, bind_keys, N, lit_keys, nbg, p, write, expect, capacity);
+    else total = ordered_join(ctx, lit_keys, nbg, bind_keys, N, p, write, expect, capacity);
     *h_num_rows = total;
+    if (write && capacity < 0 && total != expect) throw Error(QF_ERR_OVERFLOW, "output size differs from the row count");
+    if (write && capacity >= 0 && total > capacity) throw Error(QF_ERR_OVERFLOW, "output rows exceed the capacity");
   });
+}
+
+int qf_create_binding_table(qf_context* c, const qf_table* binding, const qf_table* relation,
+                            const qf_literal* literal, int32_t* const* d_out_columns, int64_t* h_num_rows) {
+  return create_binding_table(c, binding, relation, literal, d_out_columns, h_num_rows, -1);
+}
+
+int qf_create_binding_table_presized(qf_context* c, const qf_table* binding, const qf_table* relation,
+                                     const qf_literal* literal, int64_t capacity, int32_t* const* d_out_columns,
+                                     int64_t* h_num_rows) {
+  if (capacity < 0 || !d_out_columns) return QF_ERR_INVALID;
+  return create_binding_table(c, binding, relation, literal, d_out_columns, h_num_rows, capacity);
 }
 
 static int semi_join(qf_context* c, const qf_table* probe_keys, const qf_table* build_keys, uint64_t* d_out_bits,
