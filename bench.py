@@ -10,6 +10,13 @@ literals/sec of the batched form; the plain form (one launch per literal) is tim
 `--workload micro` runs the operator microbench instead: 16,777,216 x 3 binding rows against
 4,194,304 x 2 facts, keys over 1M values (uniform or Zipf 1.0), three literals.
 
+`--workload learn-kinship` / `learn-bongard` time end-to-end rule learning (host learner over the
+device backend) on the synthetic kinship forest (2,000 persons, clause length 3) and Bongard
+pictures (200,000, clause length 4), with the oracle backend timed beside it and the learnt
+clauses compared.  `--workload estimator` records join instances from refinement steps, trains
+the cardinality MLP on the device and reports q-error, presize fallback rate and the time of
+presized joins against the two-pass joins.
+
 Prints one JSON line (see the keys below).  --gpus above 1 is refused: this build runs on one
 GPU.
 """
@@ -92,7 +99,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["step", "micro"], default="step")
+    ap.add_argument("--workload", choices=["step", "micro", "learn-kinship", "learn-bongard", "estimator"],
+                    default="step")
     ap.add_argument("--dist", choices=["uniform", "zipf"], default="zipf")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-fraction", type=float, default=1.0 / 256)
@@ -109,6 +117,11 @@ def main():
     import torch
     import quickfoil_amd
     from quickfoil_amd import datagen
+
+    if args.workload.startswith("learn-"):
+        return bench_learner(args)
+    if args.workload == "estimator":
+        return bench_estimator(args)
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
@@ -229,6 +242,120 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_learner(args):
+    """End-to-end QuickFoil learning on a synthetic problem, device backend vs the oracle backend
+    (one host thread); clauses must be identical."""
+    import ctypes
+    import quickfoil_amd
+    from quickfoil_amd import datagen, learner
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if args.workload == "learn-kinship":
+        k = datagen.kinship()
+        r = k["relations"]
+        bg = [learner.Relation(n, r[n], [0] * len(r[n])) for n in ["parent", "married", "male", "female", "brother",
+                                                                  "sister"]]
+        problem = learner.Problem(learner.Relation("uncle", k["target"], [0, 0]), k["num_positive"], bg)
+        opts, name = {"maximum_clause_length": 3}, "kinship: 2000 persons, uncle/2, clause length 3"
+    else:
+        b = datagen.bongard(200000)
+        bg = [learner.Relation(n, c, t) for n, (c, t) in b["relations"].items()]
+        problem = learner.Problem(learner.Relation("picture", b["target"], [0]), b["num_positive"], bg)
+        opts, name = {"maximum_clause_length": 4}, "bongard: 200000 pictures, 7 relations, clause length 4"
+    options = learner.default_options(**opts)
+    ctx = quickfoil_amd.Context(0)
+    dev = learner.DeviceBackend(ctx, problem)
+    learner.learn(problem, dev.backend, options)  # warm-up (code paths, pools)
+    times = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        clauses = learner.learn(problem, dev.backend, options)
+        times.append(time.perf_counter() - t0)
+    dev.close()
+    lib = oracle.lib()
+    n, ar, cols, rows, types, keep = problem.relation_arrays()
+    cpu = learner.LearnerBackend()
+    lib.orc_learner_backend_create(n, ar, cols, rows, types, ctypes.byref(cpu))
+    t0 = time.perf_counter()
+    cpu_clauses = learner.learn(problem, cpu, options)
+    cpu_s = time.perf_counter() - t0
+    gpu_s = min(times)
+    out = {"metric": "end-to-end rule learning time", "value": gpu_s, "unit": "s", "n_gpus": 1, "steps": len(times),
+           "warmup": 1, "higher_is_better": False, "dtype": "int32", "data": "synthetic (seeded generators)",
+           "config": {"workload": name}, "clauses": clauses, "identical_to_oracle_backend": clauses == cpu_clauses,
+           "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": 1, "kind": "port",
+                            "sample": "the same learner over the oracle backend (reference pipeline restated, "
+                                      "1 thread), full problem"}}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_estimator(args):
+    """c5: record join instances from refinement steps (1/64 scale, varied seeds), train the MLP,
+    report q-error on held-out steps, presize fallback rate and join time against two passes."""
+    import torch
+    import quickfoil_amd
+    from quickfoil_amd import estimator
+    ctx = quickfoil_amd.Context(0)
+    t0 = time.perf_counter()
+    train = [estimator.record_step(ctx, 1.0 / 64, 100 + i, log=log)[:2] for i in range(24)]
+    X = np.concatenate([t[0] for t in train])
+    y = np.concatenate([t[1] for t in train])
+    held = [estimator.record_step(ctx, 1.0 / 64, 900 + i, log=log) for i in range(2)]
+    Xh = np.concatenate([h[0] for h in held])
+    yh = np.concatenate([h[1] for h in held])
+    rec_s = time.perf_counter() - t0
+    est = estimator.Estimator(ctx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    est.fit(X, y, epochs=60, batch=512, lr=3e-3, log=log)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    pred = est.predict(Xh)
+    q = estimator.q_error(pred, yh)
+    qi = estimator.q_error(estimator.independence_estimate(Xh), yh)
+    # Presized joins on the first held-out step against the two-pass join.
+    _, _, w = held[0]
+    dev = torch.device("cuda")
+    binding = [torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in w["binding"]]
+    rels = [[torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in r] for r in w["relations"]]
+    picks = [i for i in range(len(w["literals"])) if yh[i] < 24][:: max(1, len(w["literals"]) // 300)]
+    fallbacks, identical = 0, True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in picks:
+        rel, vars_ = w["literals"][i]
+        ctx.create_binding_table(binding, rels[rel], vars_)
+    torch.cuda.synchronize()
+    two_pass_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in picks:
+        rel, vars_ = w["literals"][i]
+        _, fb = ctx.create_binding_table_presized(binding, rels[rel], vars_, estimator.presize_rows(pred[i]))
+        fallbacks += int(fb)
+    torch.cuda.synchronize()
+    presized_s = time.perf_counter() - t0
+    for i in picks[:: max(1, len(picks) // 20)]:
+        rel, vars_ = w["literals"][i]
+        a = ctx.create_binding_table(binding, rels[rel], vars_)
+        b, _ = ctx.create_binding_table_presized(binding, rels[rel], vars_, estimator.presize_rows(pred[i]))
+        identical = identical and all(torch.equal(x, z) for x, z in zip(a, b))
+    out = {"metric": "join-cardinality estimator q-error (held-out joins)", "value": float(np.median(q)),
+           "unit": "q-error p50", "n_gpus": 1, "higher_is_better": False, "dtype": "bf16 (fp32 accumulate)",
+           "data": "synthetic (joins recorded from refinement steps at 1/64 scale)",
+           "config": {"workload": "estimator: MLP 40-64-64-1, %d training joins from 24 steps, %d held-out"
+                                  % (len(y), len(yh))},
+           "q_error_p50": float(np.median(q)), "q_error_p99": float(np.percentile(q, 99)),
+           "independence_q_error_p50": float(np.median(qi)), "independence_q_error_p99": float(np.percentile(qi, 99)),
+           "presized_joins": len(picks), "fallback_rate": fallbacks / max(1, len(picks)),
+           "two_pass_join_s": two_pass_s, "presized_join_s": presized_s,
+           "presize_speedup": two_pass_s / presized_s if presized_s > 0 else None,
+           "identical_outputs": identical, "record_s": rec_s, "train_s": train_s}
+    print(json.dumps(out), flush=True)
+    est.close()
     ctx.close()
 
 
