@@ -83,6 +83,9 @@ struct DevUnit {
   int ngroups;
   int nsub;
   int jcol;               // the join column (subset 0's key)
+  uint32_t orphan;        // bit s: subset s has a literal without a parent mask
+  uint64_t parmask[EV_MAXSUB];  // subset s: the mask slots of its literals' parents
+  int8_t slot_group[EV_MAXMASK];  // the unit group of the literal owning each mask slot
   int ug_of_bg[EV_MAXG];  // the unit's group for each group of its batch, or -1
   DevUGroup g[EV_MAXG];
   DevSubset sub[EV_MAXSUB];
@@ -312,6 +315,11 @@ struct ItemShared {
   int blk_start[EV_MAXG + 1];  // join phase: 64-row blocks of all rows, per segment
   int ent_start[EV_MAXG + 1];  // subset phase: (literal, block of hit rows) entries, per segment
   int need[EV_MAXSUB];         // subset s has a literal whose parent mask is not all zero
+  int seg_of_ug[EV_MAXG];      // the current unit's segment for each of its groups, or -1
+  int64_t u_p0[EV_MAXBU];      // per unit of the batch: first fact of cell q, fact count, heavy ordinal
+  int u_np[EV_MAXBU];
+  int u_hq[EV_MAXBU];
+  int8_t u_ug[EV_MAXBU][EV_MAXG];  // per unit and batch group: the unit's group, or -1
   int nseg;
 };
 
@@ -392,6 +400,34 @@ __device__ inline uint32_t lane_prefix(uint64_t m, int lane) {
 // whose join value occurs in the partition are listed per segment in `cidx` (segment-relative
 // row numbers); every literal of the unit binds the join column, so the other rows match no
 // literal.  The literals keyed on the join column alone (subset 0) are counted here.
+// Two single-column probes of an LDS table interleaved, so that their slot and key reads are in
+// flight together.  Finished lanes keep reading in-range words.
+__device__ inline void lds_probe1_x2(const uint32_t* tbl, uint32_t cap, const int32_t* pkey, int32_t va, int32_t vb,
+                                     bool live_a, bool live_b, uint32_t& ca, uint32_t& cb) {
+  const uint32_t ha = tuple_hash(&va, 1), hb = tuple_hash(&vb, 1);
+  const uint32_t fa = ha >> EV_FP_SHIFT, fb = hb >> EV_FP_SHIFT;
+  uint32_t sa = ha & (cap - 1), sb = hb & (cap - 1);
+  ca = cb = 0;
+  while (live_a || live_b) {
+    const uint32_t wa = tbl[sa], wb = tbl[sb];
+    const int ra = (int)((wa >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1, rb = (int)((wb >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1;
+    const int32_t ka = pkey[ra < 0 ? 0 : ra], kb = pkey[rb < 0 ? 0 : rb];
+    const bool hit_a = wa != 0 && (wa >> EV_FP_SHIFT) == fa && ka == va;
+    const bool hit_b = wb != 0 && (wb >> EV_FP_SHIFT) == fb && kb == vb;
+    if (live_a && hit_a) ca = wa & EV_FIELD_MASK;
+    if (live_b && hit_b) cb = wb & EV_FIELD_MASK;
+    live_a = live_a && wa != 0 && !hit_a;
+    live_b = live_b && wb != 0 && !hit_b;
+    sa = (sa + 1) & (cap - 1);
+    sb = (sb + 1) & (cap - 1);
+  }
+}
+
+// Join phase of an item: every staged row probes the table of the join column alone.  Rows
+// whose join value occurs in the partition are listed per segment in `cidx` (segment-relative
+// row numbers); every literal of the unit binds the join column, so the other rows match no
+// literal.  The literals keyed on the join column alone (subset 0) are counted here.  A wave
+// takes two 64-row blocks of a segment per step.
 template <bool kLds>
 __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, const int32_t* tstage, uint16_t* cidx,
                            const uint32_t* tbl, const uint32_t* cnt, const int32_t* keys, uint32_t cap,
@@ -403,7 +439,7 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
   const int tile = a.tile_rows;
   const int32_t* tid_col = tstage + a.ntc * tile;
   cptr<DevLit> lits = as_const(a.lits);
-  const int poff = U->jcol * np;
+  const int32_t* pkey = pstage + U->jcol * np;
   int sg = 0;
   while (e0 >= uni(sh.blk_start[sg + 1])) ++sg;
   int blk = e0 - uni(sh.blk_start[sg]);
@@ -411,40 +447,50 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
   int g = uni(sh.seg_ug[sg]);
   int jt = U->g[g].jtcol * tile, l0 = U->g[g].sbeg[0], l1 = U->g[g].sbeg[1];
   uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
-  for (int e = e0; e < e1; ++e) {
-    const int rloc = blk * 64 + lane;
-    const bool valid = rloc < seg_rows;
-    const int row = seg_off + (valid ? rloc : 0);
-    uint32_t c = 0;
-    if (valid) {
-      const int32_t v = tstage[jt + row];
-      if (kLds) c = lds_probe<1>(tbl, cap, pstage, &poff, &v, 1);
-      else c = table_probe(cnt, keys, cap, &v, 1);
+  for (int e = e0; e < e1;) {
+    const bool two = e + 1 < e1 && blk + 1 < nblk;
+    const int ra = blk * 64 + lane, rb = ra + 64;
+    const bool va_ok = ra < seg_rows, vb_ok = two && rb < seg_rows;
+    const int row_a = seg_off + (va_ok ? ra : 0), row_b = seg_off + (vb_ok ? rb : 0);
+    const int32_t va = tstage[jt + row_a], vb = tstage[jt + row_b];
+    uint32_t ca = 0, cb = 0;
+    if (kLds) {
+      lds_probe1_x2(tbl, cap, pkey, va, vb, va_ok, vb_ok, ca, cb);
+    } else {
+      if (va_ok) ca = table_probe(cnt, keys, cap, &va, 1);
+      if (vb_ok) cb = table_probe(cnt, keys, cap, &vb, 1);
     }
-    const uint64_t m = __ballot(c != 0);
-    if (m) {
+    const uint64_t ma = __ballot(ca != 0), mb = __ballot(cb != 0);
+    if (ma | mb) {
       int base = 0;
-      if (lane == 0) base = atomicAdd(&sh.seg_hits[sg], __popcll(m));
+      if (lane == 0) base = atomicAdd(&sh.seg_hits[sg], __popcll(ma) + __popcll(mb));
       base = __shfl(base, 0, 64);
-      if (c != 0) cidx[seg_off + base + (int)lane_prefix(m, lane)] = (uint16_t)rloc;
+      if (ca != 0) cidx[seg_off + base + (int)lane_prefix(ma, lane)] = (uint16_t)ra;
+      if (cb != 0) cidx[seg_off + base + __popcll(ma) + (int)lane_prefix(mb, lane)] = (uint16_t)rb;
       if (l1 > l0) {
-        const int32_t tid = tid_col[row];
-        const bool pos = tid < a.num_positive;
-        cp += __popcll(__ballot(c != 0 && pos));
-        cn += __popcll(__ballot(c != 0 && !pos));
-        lbp += pos ? c : 0u;
-        lbn += pos ? 0u : c;
-        if (a.bits != nullptr && c != 0)
-          for (int li = l0; li < l1; ++li)
-            atomicOr(&a.bits[(int64_t)lits[li].slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+        const int32_t ta = tid_col[row_a], tb = tid_col[row_b];
+        const bool pa = ta < a.num_positive, pb = tb < a.num_positive;
+        cp += __popcll(__ballot(ca != 0 && pa)) + __popcll(__ballot(cb != 0 && pb));
+        cn += __popcll(__ballot(ca != 0 && !pa)) + __popcll(__ballot(cb != 0 && !pb));
+        lbp += (pa ? ca : 0u) + (pb ? cb : 0u);
+        lbn += (pa ? 0u : ca) + (pb ? 0u : cb);
+        if (a.bits != nullptr)
+          for (int li = l0; li < l1; ++li) {
+            unsigned long long* bits = a.bits + (int64_t)lits[li].slot * a.bit_words;
+            if (ca != 0) atomicOr(&bits[ta >> 6], 1ull << (ta & 63));
+            if (cb != 0) atomicOr(&bits[tb >> 6], 1ull << (tb & 63));
+          }
       }
     }
-    if (++blk == nblk) {  // the segment's blocks end
+    const int step = two ? 2 : 1;
+    e += step;
+    blk += step;
+    if (blk == nblk) {  // the segment's blocks end
       flush_counts(a, lits, l0, l1, cp, cn, lbp, lbn, lane);
       blk = 0;
-      if (e + 1 < e1) {
+      if (e < e1) {
         do ++sg;
-        while (e + 1 >= uni(sh.blk_start[sg + 1]));  // segments of groups the unit lacks have no blocks
+        while (e >= uni(sh.blk_start[sg + 1]));  // segments of groups the unit lacks have no blocks
         seg_rows = uni(sh.seg_rows[sg]);
         seg_off = uni(sh.seg_off[sg]);
         nblk = (seg_rows + 63) >> 6;
@@ -604,7 +650,57 @@ __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const Ite
   }
 }
 
-__global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
+// Exclusive prefix sums across the 64 lanes of a wave.
+__device__ inline int wave_excl_prefix(int v, int lane) {
+  int x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x - v;
+}
+__device__ inline int64_t wave_excl_prefix64(int64_t v, int lane) {
+  int64_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x - v;
+}
+
+// Facts of an LDS partition, prefetched into registers while the previous unit is evaluated:
+// thread t holds flattened words t + EV_THREADS * j of the staged layout [column][row].
+#ifndef QF_EV_PF
+#define QF_EV_PF 3
+#endif
+constexpr int EV_PF = QF_EV_PF;  // fact words per thread carried in registers to the next unit
+struct FactPrefetch {
+  int32_t v[EV_PF];
+  int unit = -1;
+  int64_t q = -1;
+};
+
+__device__ inline void prefetch_facts(cptr<DevUnit> U, int unit, int64_t q, int64_t p0, int np, FactPrefetch& pf) {
+  const int words = np * U->arity;
+#pragma unroll
+  for (int j = 0; j < EV_PF; ++j) {
+    const int f = threadIdx.x + EV_THREADS * j;
+    if (f < words) {
+      const int c = f / np, i = f - c * np;
+      pf.v[j] = U->pcols[c][p0 + i];
+    }
+  }
+  pf.unit = unit;
+  pf.q = q;
+}
+
+// Three 512-thread blocks share a CU when their LDS fits (about 50 KB each); that takes at most
+// 80 VGPRs per lane (6 waves per SIMD), which the attribute asks the register allocator for.
+#ifndef QF_EV_WAVES_PER_EU
+#define QF_EV_WAVES_PER_EU 6
+#endif
+__global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_EV_WAVES_PER_EU))) void k_eval(
+    EvalArgs a) {
   extern __shared__ uint32_t lds[];
   uint32_t* tbl = lds;                                                                 // EV_TBL_WORDS
   int32_t* pstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS);                   // EV_PST_WORDS
@@ -619,6 +715,7 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
   const int ntc = a.ntc;
   cptr<DevItem> items = as_const(a.items);
   cptr<DevLit> lits = as_const(a.lits);
+  FactPrefetch pf;
 
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
     const int item_batch = items[it].batch, item_chunk = items[it].chunk;
@@ -626,26 +723,46 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
     cptr<DevBatch> B = as_const(a.batches) + item_batch;
 
     __syncthreads();  // the previous item is done with LDS
-    // Segments: the chunk's rows over the batch's join variables, in group order.
-    if (threadIdx.x == 0) {
+    // Segments: the chunk's rows over the batch's join variables, in group order (wave 0, one
+    // lane per group).
+    if (wave == 0) {
       const int64_t c0 = (int64_t)item_chunk * a.tile_rows, c1 = c0 + a.tile_rows;
-      int64_t run = 0;
-      int ns = 0, off = 0;
-      for (int g = 0; g < B->ngroups; ++g) {
-        const int64_t t0 = B->toff[g][q << B->shift], t1 = B->toff[g][(q + 1) << B->shift];
-        const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
-        if (hi_ > lo_) {
-          sh.seg_group[ns] = g;
-          sh.seg_off[ns] = off;
-          sh.seg_rows[ns] = (int)(hi_ - lo_);
-          sh.seg_src[ns] = t0 + (lo_ - run);
-          off += (int)(hi_ - lo_);
-          ++ns;
-        }
-        run += t1 - t0;
+      const int g = lane;
+      const bool ok = g < B->ngroups;
+      int64_t t0 = 0, t1 = 0;
+      if (ok) {
+        t0 = B->toff[g][q << B->shift];
+        t1 = B->toff[g][(q + 1) << B->shift];
       }
-      sh.seg_off[ns] = off;
-      sh.nseg = ns;
+      const int64_t run = wave_excl_prefix64(t1 - t0, lane);
+      const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
+      const int n = (ok && hi_ > lo_) ? (int)(hi_ - lo_) : 0;
+      const uint64_t has = __ballot(n > 0);
+      const int idx = __popcll(has & ((1ull << lane) - 1ull));
+      const int off = wave_excl_prefix(n, lane);
+      if (n > 0) {
+        sh.seg_group[idx] = g;
+        sh.seg_off[idx] = off;
+        sh.seg_rows[idx] = n;
+        sh.seg_src[idx] = t0 + (lo_ - run);
+      }
+      if (lane == 63) {
+        sh.seg_off[__popcll(has)] = off + n;
+        sh.nseg = __popcll(has);
+      }
+    } else if (wave == 1) {  // the units' partitions at q, one lane per unit
+      if (lane < B->nunits) {
+        cptr<DevUnit> V = as_const(a.units) + B->unit[lane];
+        const int64_t v0 = V->poff[q];
+        sh.u_p0[lane] = v0;
+        sh.u_np[lane] = (int)(V->poff[q + 1] - v0);
+        sh.u_hq[lane] = V->hq[q];
+      }
+    } else if (wave == 2) {  // the units' group maps, one lane per (unit, batch group)
+      for (int l = lane; l < B->nunits * B->ngroups; l += 64) {
+        const int k = l / B->ngroups, g = l - k * B->ngroups;
+        sh.u_ug[k][g] = (int8_t)(as_const(a.units) + B->unit[k])->ug_of_bg[g];
+      }
     }
     __syncthreads();
     const int nseg = uni(sh.nseg);
@@ -663,30 +780,59 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
 
    for (int k = 0; k < B->nunits; ++k) {
     cptr<DevUnit> U = as_const(a.units) + B->unit[k];
-    const int64_t p0 = U->poff[q];
-    const int np = (int)(U->poff[q + 1] - p0);
+    const int64_t p0 = uni64(sh.u_p0[k]);
+    const int np = uni(sh.u_np[k]);
     if (np == 0) continue;
-    const int hq = U->hq[q];
+    const int hq = uni(sh.u_hq[k]);
     __syncthreads();  // staged rows ready; the previous unit is done with the table, facts, lists and masks
-    if (threadIdx.x == 0) {
-      int blocks = 0;
-      for (int sg2 = 0; sg2 < nseg; ++sg2) {
-        const int ug = U->ug_of_bg[sh.seg_group[sg2]];
-        sh.seg_ug[sg2] = ug;
-        sh.seg_hits[sg2] = 0;
-        sh.blk_start[sg2] = blocks;
-        if (ug >= 0) blocks += (sh.seg_rows[sg2] + 63) >> 6;
+    if (wave == 0) {  // segment map of this unit, one lane per segment
+      const bool ok = lane < nseg;
+      int ug = -1, blocks = 0;
+      if (ok) {
+        ug = sh.u_ug[k][sh.seg_group[lane]];
+        blocks = ug >= 0 ? (sh.seg_rows[lane] + 63) >> 6 : 0;
+        sh.seg_ug[lane] = ug;
+        sh.seg_hits[lane] = 0;
       }
-      sh.blk_start[nseg] = blocks;
-      for (int s = 0; s < EV_MAXSUB; ++s) sh.need[s] = 0;
+      const int bs = wave_excl_prefix(blocks, lane);
+      if (ok) sh.blk_start[lane] = bs;
+      if (lane == 63) sh.blk_start[nseg] = bs + blocks;
+      if (lane < EV_MAXSUB) sh.need[lane] = 0;
+      if (lane < EV_MAXG) sh.seg_of_ug[lane] = -1;
+      if (ok && ug >= 0) sh.seg_of_ug[ug] = lane;
     }
-    // The partition's facts (all relation columns) for an LDS table.
+    // The partition's facts (all relation columns) for an LDS table, from the registers the
+    // previous unit prefetched them into when it could.
     if (hq < 0) {
-      for (int c = 0; c < U->arity; ++c) {
-        const int32_t* src = U->pcols[c] + p0;
-        for (int i = threadIdx.x; i < np; i += EV_THREADS) pstage[c * np + i] = src[i];
+      const int unit = B->unit[k];
+      if (pf.unit == unit && pf.q == q) {
+        const int words = np * U->arity;
+#pragma unroll
+        for (int j = 0; j < EV_PF; ++j) {
+          const int f = threadIdx.x + EV_THREADS * j;
+          if (f < words) pstage[f] = pf.v[j];
+        }
+        for (int f = threadIdx.x + EV_THREADS * EV_PF; f < words; f += EV_THREADS) {
+          const int c = f / np, i = f - c * np;
+          pstage[f] = U->pcols[c][p0 + i];
+        }
+      } else {
+        for (int c = 0; c < U->arity; ++c) {
+          const int32_t* src = U->pcols[c] + p0;
+          for (int i = threadIdx.x; i < np; i += EV_THREADS) pstage[c * np + i] = src[i];
+        }
       }
       for (uint32_t i = threadIdx.x, cap = table_cap_lds(np); i < cap; i += EV_THREADS) tbl[i] = 0;
+    }
+    // Start loading the next unit's facts; they arrive while this unit is evaluated.
+    {
+      int nk2 = -1;
+      for (int j = k + 1; j < B->nunits && nk2 < 0; ++j)
+        if (uni(sh.u_np[j]) > 0 && uni(sh.u_hq[j]) < 0) nk2 = j;
+      if (nk2 >= 0) {
+        cptr<DevUnit> V = as_const(a.units) + B->unit[nk2];
+        prefetch_facts(V, B->unit[nk2], q, uni64(sh.u_p0[nk2]), uni(sh.u_np[nk2]), pf);
+      }
     }
     __syncthreads();  // segment map, facts and cleared table ready
 
@@ -706,25 +852,26 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval(EvalArgs a) {
 
     for (int s = 1; s < U->nsub; ++s) {
       __syncthreads();  // hit lists / previous subset's probes and masks done
-      if (threadIdx.x == 0) {
-        int e = 0;
-        for (int sg2 = 0; sg2 < nseg; ++sg2) {
-          const int g = sh.seg_ug[sg2];
-          sh.ent_start[sg2] = e;
-          if (g >= 0) e += (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_hits[sg2] + 63) >> 6);
-        }
-        sh.ent_start[nseg] = e;
+      if (wave == 0) {  // entry list, one lane per segment
+        const bool ok = lane < nseg;
+        const int g = ok ? sh.seg_ug[lane] : -1;
+        const int e = g >= 0 ? (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_hits[lane] + 63) >> 6) : 0;
+        const int es = wave_excl_prefix(e, lane);
+        if (ok) sh.ent_start[lane] = es;
+        if (lane == 63) sh.ent_start[nseg] = es + e;
       }
-      // Whether any literal of the subset can match: one without a parent mask, or one whose
-      // parent matched some row.
-      for (int sg2 = 0; sg2 < nseg; ++sg2) {
-        const int g = uni(sh.seg_ug[sg2]);
-        if (g < 0) continue;
-        const int la = U->g[g].sbeg[s], nl = U->g[g].sbeg[s + 1] - la;
-        const int nb = (uni(sh.seg_hits[sg2]) + 63) >> 6;
-        for (int i = threadIdx.x; i < nl * nb; i += EV_THREADS) {
-          const int pm = lits[la + i / nb].pmask;
-          if (pm < 0 || masks[pm * a.mstride + i % nb] != 0) sh.need[s] = 1;
+      // Whether any literal of the subset can match: one without a parent mask, or a parent
+      // mask with a row set (masks left by earlier items only add work, never skip it).
+      if ((U->orphan >> s) & 1u) {
+        if (threadIdx.x == 0) sh.need[s] = 1;
+      } else {
+        // Each parent's mask words of its segment's blocks of listed rows.
+        const uint64_t pm = U->parmask[s];
+        for (int i = threadIdx.x; i < EV_MAXMASK * a.mstride; i += EV_THREADS) {
+          const int slot = i / a.mstride, b = i - slot * a.mstride;
+          if (!((pm >> slot) & 1ull)) continue;
+          const int sg = sh.seg_of_ug[U->slot_group[slot]];
+          if (sg >= 0 && b < ((sh.seg_hits[sg] + 63) >> 6) && masks[i] != 0) sh.need[s] = 1;
         }
       }
       if (hq < 0)
@@ -1080,7 +1227,9 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
                      [](const SubKey& x, const SubKey& y) { return x.first.size() < y.first.size(); });
     QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one unit");
     du.jcol = jc;
+    for (int m = 0; m < EV_MAXMASK; ++m) du.slot_group[m] = -1;
     int unit_lits = 0, nmask = 0;
+    const size_t unit_first = P.lits.size();
     for (int gi = 0; gi < du.ngroups; ++gi) {
       const int vi = var_index(S.groups[gi]->join_var);
       DevUGroup& G = du.g[gi];
@@ -1130,6 +1279,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
           DevLit& pd = P.lits[first + j];
           if (pd.mslot < 0) {
             if (nmask >= EV_MAXMASK) break;
+            du.slot_group[nmask] = (int8_t)gi;
             pd.mslot = nmask++;
           }
           P.lits[first + i].pmask = pd.mslot;
@@ -1138,6 +1288,13 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     }
     QF_REQUIRE(unit_lits <= EV_MAXLU, "too many literals in one unit");
     P.max_mask = std::max(P.max_mask, nmask);
+    du.orphan = 0;
+    for (int s2 = 0; s2 < EV_MAXSUB; ++s2) du.parmask[s2] = 0;
+    for (size_t i = unit_first; i < P.lits.size(); ++i) {
+      const DevLit& dl = P.lits[i];
+      if (dl.pmask < 0) du.orphan |= 1u << dl.subset;
+      else du.parmask[dl.subset] |= 1ull << dl.pmask;
+    }
     du.nsub = (int)subsets.size();
     for (int s = 0; s < du.nsub; ++s) {
       du.sub[s].nk = (int)subsets[s].first.size();
