@@ -295,7 +295,9 @@ struct EvalArgs {
   const int64_t* num_items;
   int ntc;                // staged binding columns (the row id follows)
   int num_positive;  // binding rows are below 2^31
-  unsigned long long* counts_out;  // num_slots x 4
+  unsigned long long* counts_out;  // ncopies x num_slots x 4: block b adds into copy b % ncopies
+  int64_t copy_words;              // num_slots x 4
+  int ncopies;
   unsigned long long* bits;        // num_slots x bit_words, or null
   int64_t bit_words;
   int tile_rows;
@@ -716,6 +718,9 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
   cptr<DevItem> items = as_const(a.items);
   cptr<DevLit> lits = as_const(a.lits);
   FactPrefetch pf;
+  // This block's copy of the counters: blocks spread their atomics over separate copies, so that
+  // few of them queue on one address.
+  a.counts_out += (int64_t)(blockIdx.x % a.ncopies) * a.copy_words;
 
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
     const int item_batch = items[it].batch, item_chunk = items[it].chunk;
@@ -888,6 +893,15 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
     }
    }
   }
+}
+
+// Sums the blocks' counter copies: out[w] = sum over c of copies[c * words + w].
+__global__ void k_sum_copies(const uint64_t* copies, int ncopies, int64_t words, uint64_t* out) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= words) return;
+  uint64_t sum = 0;
+  for (int c = 0; c < ncopies; ++c) sum += copies[(int64_t)c * words + w];
+  out[w] = sum;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1104,6 +1118,8 @@ struct StepPlan {
   std::vector<int64_t> unit_cells;            // per launch-local prefix, concatenated per launch
   std::vector<int64_t> launch_items;          // per launch: upper bound of its work items
   DevBuf d_units, d_batches, d_lits, d_unit_cells, d_cell_counts, d_cell_starts, d_items, d_num_items, d_counts;
+  DevBuf d_copies;  // ncopies x num_slots x 4 partial counters, summed into d_counts after the launches
+  int ncopies = 1;
   int64_t max_cells = 0, max_items = 0;
   size_t lds_bytes = 0;
   int max_blocks = 0;
@@ -1409,13 +1425,17 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   int per_cu = 0;
   QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eval, EV_THREADS, P.lds_bytes));
   P.max_blocks = ctx->num_cus * std::max(1, per_cu);
+  // Counter copies: up to one per CU, within 32 MiB.
+  const int64_t copy_bytes = (int64_t)std::max(1, num_slots) * 4 * (int64_t)sizeof(uint64_t);
+  P.ncopies = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(ctx->num_cus, P.max_blocks), (32ll << 20) / copy_bytes));
+  P.d_copies = DevBuf(ctx, (size_t)(P.ncopies * copy_bytes));
   P.bit_words = ceil_div(binding.num_rows, 64);
 }
 
 // Device work of one step, all on the context stream (capturable: kernels and device copies).
 static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_positive, StepPlan& P, uint64_t* d_bits,
                          uint64_t* h_out) {
-  fill_u64(ctx, P.d_counts.as<uint64_t>(), 0, (int64_t)P.num_slots * 4);
+  fill_u64(ctx, P.d_copies.as<uint64_t>(), 0, (int64_t)P.ncopies * P.num_slots * 4);
   if (d_bits && P.num_slots > 0) fill_u64(ctx, d_bits, 0, (int64_t)P.num_slots * P.bit_words);
   if (!P.units.empty() && binding.num_rows > 0) {
     for (size_t v = 0; v < P.join_vars.size(); ++v) {
@@ -1427,7 +1447,9 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
     a.lits = P.d_lits.as<DevLit>();
     a.ntc = (int)P.bcols.size();
     a.num_positive = (int)num_positive;
-    a.counts_out = reinterpret_cast<unsigned long long*>(P.d_counts.get());
+    a.counts_out = reinterpret_cast<unsigned long long*>(P.d_copies.get());
+    a.copy_words = (int64_t)P.num_slots * 4;
+    a.ncopies = P.ncopies;
     a.bits = reinterpret_cast<unsigned long long*>(d_bits);
     a.bit_words = P.bit_words;
     a.tile_rows = P.tile;
@@ -1464,6 +1486,12 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
       hipLaunchKernelGGL(k_eval, dim3(blocks), dim3(EV_THREADS), P.lds_bytes, ctx->stream, a);
       QF_CHECK_HIP(hipGetLastError());
     }
+  }
+  if (P.num_slots > 0) {
+    const int64_t words = (int64_t)P.num_slots * 4;
+    hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)ceil_div(words, 256)), dim3(256), 0, ctx->stream,
+                       P.d_copies.as<uint64_t>(), P.ncopies, words, P.d_counts.as<uint64_t>());
+    QF_CHECK_HIP(hipGetLastError());
   }
   QF_CHECK_HIP(hipMemcpyAsync(h_out, P.d_counts.get(), (size_t)P.num_slots * 4 * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, ctx->stream));
