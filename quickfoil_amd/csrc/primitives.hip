@@ -1,10 +1,11 @@
 // Device primitives: memory pool, fills, exclusive scan, and the stable radix partition that
 // RadixPartition, the chained-table builds and the evaluator's co-partitioning all use.
 //
-// Stable partition of one tile (2048 rows, 4 waves of 64 lanes x 8 rows) per workgroup:
+// Stable partition of one tile (2048 rows, 8 waves of 64 lanes x 4 rows) per workgroup:
 //   1. histogram kernel: per-tile digit counts in LDS, written bin-major to HBM;
 //   2. exclusive scan over (bin, tile) gives each tile's output base per bin;
-//   3. scatter kernel: per-wave LDS histograms and wave-level prefix sums give every row its
+//   3. scatter kernel: the tile's rows are loaded into registers first (every column's loads in
+//      flight together), per-wave LDS histograms and wave-level prefix sums give every row its
 //      stable rank (peers found with one ballot per digit bit), rows are staged in LDS in
 //      bin order, then written out so consecutive lanes store consecutive addresses of a bin.
 // Digits wider than 8 bits are done as stable LSD passes of <= 8 bits.
@@ -197,9 +198,9 @@ void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n
 // ------------------------------------------------------------------------------------------
 // Stable radix partition.
 
-constexpr int PT_THREADS = 256;
+constexpr int PT_THREADS = 512;
 constexpr int PT_WAVES = PT_THREADS / 64;
-constexpr int PT_IPT = 8;
+constexpr int PT_IPT = 4;
 constexpr int PT_TILE = PT_THREADS * PT_IPT;
 constexpr int PT_MAXCOLS = 8;
 constexpr int PT_MAXBINS = 256;
@@ -228,97 +229,122 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_hist(PassArgs a, int64_t* c
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * PT_TILE;
   const int32_t* key = a.in[a.key_col];
+  uint32_t d[PT_IPT];
+#pragma unroll
   for (int i = 0; i < PT_IPT; ++i) {
     const int64_t row = base + (int64_t)i * PT_THREADS + threadIdx.x;
-    if (row < a.n) atomicAdd(&hist[digit_of(key[row], a.mixed, a.shift, a.bits)], 1u);
+    d[i] = row < a.n ? digit_of(key[row], a.mixed, a.shift, a.bits) : PT_MAXBINS;
   }
+#pragma unroll
+  for (int i = 0; i < PT_IPT; ++i)
+    if (d[i] < PT_MAXBINS) atomicAdd(&hist[d[i]], 1u);
   __syncthreads();
   for (int b = threadIdx.x; b < nb; b += PT_THREADS) counts[(int64_t)b * a.num_tiles + blockIdx.x] = hist[b];
 }
 
-__global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassArgs a, const int64_t* bases) {
-  __shared__ uint32_t whist[PT_WAVES][PT_MAXBINS];
-  __shared__ uint32_t wstart[PT_WAVES][PT_MAXBINS];
-  __shared__ uint32_t lstart[PT_MAXBINS + 1];
-  __shared__ uint32_t scan_tmp[PT_MAXBINS];
-  __shared__ int32_t stage[PT_MAXCOLS][PT_TILE];
+// LDS of the scatter kernel beyond its static arrays: the staged tile (ncols x PT_TILE words)
+// and one digit byte per row.
+static size_t scatter_lds_bytes(int ncols) { return (size_t)ncols * PT_TILE * 4 + PT_TILE; }
+
+// 6 waves per SIMD (at most 80 VGPRs): three blocks per CU, as their LDS allows for up to 5 columns.
+__global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_part_scatter(
+    PassArgs a, const int64_t* bases) {
+  extern __shared__ int32_t stage[];                // ncols x PT_TILE rows in bin order, then digits
+  __shared__ uint32_t wrun[PT_WAVES][PT_MAXBINS];  // per-wave counts, then per-wave running starts
+  __shared__ int64_t gbase[PT_MAXBINS];            // output row of bin b's first tile row, minus its tile start
+  __shared__ uint32_t wtot[PT_MAXBINS / 64];
+  uint8_t* dig8 = reinterpret_cast<uint8_t*>(stage + a.ncols * PT_TILE);
 
   const int nb = 1 << a.bits;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  for (int b = threadIdx.x; b < nb; b += PT_THREADS)
-    for (int ww = 0; ww < PT_WAVES; ++ww) whist[ww][b] = 0;
-  __syncthreads();
+  for (int b = threadIdx.x; b < PT_WAVES * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0])[b] = 0;
 
+  // Each wave owns PT_IPT consecutive 64-row groups of the tile, so (wave, group, lane) is row order.
   const int64_t tile_base = (int64_t)blockIdx.x * PT_TILE;
   const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
   const int32_t* key = a.in[a.key_col];
-  uint32_t dig[PT_IPT];
+  int32_t v[PT_IPT][PT_MAXCOLS];
+  uint32_t d[PT_IPT];
 #pragma unroll
   for (int i = 0; i < PT_IPT; ++i) {
     const int64_t row = wave_base + (int64_t)i * 64 + lane;
-    dig[i] = row < a.n ? digit_of(key[row], a.mixed, a.shift, a.bits) : 0u;
-    if (row < a.n) atomicAdd(&whist[w][dig[i]], 1u);
+    const bool ok = row < a.n;
+    d[i] = ok ? digit_of(key[row], a.mixed, a.shift, a.bits) : 0u;
+#pragma unroll
+    for (int c = 0; c < PT_MAXCOLS; ++c)
+      if (c < a.ncols) v[i][c] = c == a.tid_col ? (int32_t)row : (ok ? a.in[c][row] : 0);
   }
+  __syncthreads();  // counts cleared
+#pragma unroll
+  for (int i = 0; i < PT_IPT; ++i)
+    if (wave_base + (int64_t)i * 64 + lane < a.n) atomicAdd(&wrun[w][d[i]], 1u);
   __syncthreads();
 
-  // Tile totals per bin, exclusive scan over bins, per-wave starts.
-  uint32_t tot = 0;
-  if (threadIdx.x < nb)
-    for (int ww = 0; ww < PT_WAVES; ++ww) tot += whist[ww][threadIdx.x];
-  scan_tmp[threadIdx.x] = threadIdx.x < nb ? tot : 0;
-  __syncthreads();
-  for (int off = 1; off < PT_THREADS; off <<= 1) {
-    uint32_t add = threadIdx.x >= off ? scan_tmp[threadIdx.x - off] : 0;
-    __syncthreads();
-    scan_tmp[threadIdx.x] += add;
-    __syncthreads();
+  // Bin totals, their exclusive scan (one lane per bin, 64 bins per wave), per-wave starts.
+  uint32_t tot = 0, incl = 0;
+  if (threadIdx.x < PT_MAXBINS) {
+    if ((int)threadIdx.x < nb)
+      for (int ww = 0; ww < PT_WAVES; ++ww) tot += wrun[ww][threadIdx.x];
+    incl = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+    if (lane == 63) wtot[w] = incl;
   }
-  if (threadIdx.x < nb) {
-    const uint32_t excl = scan_tmp[threadIdx.x] - tot;
-    lstart[threadIdx.x] = excl;
+  __syncthreads();
+  if ((int)threadIdx.x < nb) {
+    uint32_t excl = incl - tot;
+    for (int ww = 0; ww < w; ++ww) excl += wtot[ww];
     uint32_t run = excl;
     for (int ww = 0; ww < PT_WAVES; ++ww) {
-      wstart[ww][threadIdx.x] = run;
-      run += whist[ww][threadIdx.x];
+      const uint32_t c = wrun[ww][threadIdx.x];
+      wrun[ww][threadIdx.x] = run;
+      run += c;
     }
+    gbase[threadIdx.x] = bases[(int64_t)threadIdx.x * a.num_tiles + blockIdx.x] - (int64_t)excl;
   }
-  if (threadIdx.x == 0) lstart[nb] = scan_tmp[PT_THREADS - 1];
   __syncthreads();
 
-  // Stable ranks within the wave, in row order, staged in LDS in bin order.
+  // Stable ranks within the wave (peers: lanes of the group with the same digit, one ballot per
+  // digit bit), rows staged in LDS in bin order.
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
   for (int i = 0; i < PT_IPT; ++i) {
     const int64_t row = wave_base + (int64_t)i * 64 + lane;
     const bool valid = row < a.n;
     const uint64_t vmask = __ballot(valid);
     if (vmask == 0) break;
-    const uint32_t d = dig[i];
     uint64_t peers = vmask;
     for (int k = 0; k < a.bits; ++k) {
-      const bool bit = (d >> k) & 1u;
+      const bool bit = (d[i] >> k) & 1u;
       const uint64_t bk = __ballot(bit);
       peers &= bit ? bk : ~bk;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
     uint32_t pos = 0;
-    if (valid) pos = wstart[w][d] + rank;
+    if (valid) pos = wrun[w][d[i]] + rank;
     __builtin_amdgcn_wave_barrier();
     if (valid) {
-      for (int c = 0; c < a.ncols; ++c)
-        stage[c][pos] = (c == a.tid_col) ? (int32_t)row : a.in[c][row];
+#pragma unroll
+      for (int c = 0; c < PT_MAXCOLS; ++c)
+        if (c < a.ncols) stage[c * PT_TILE + pos] = v[i][c];
+      dig8[pos] = (uint8_t)d[i];
     }
     __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) wstart[w][d] += (uint32_t)__popcll(peers);
+    if (valid && rank == 0) wrun[w][d[i]] += (uint32_t)__popcll(peers);
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
 
-  const int64_t tile_rows = min((int64_t)PT_TILE, a.n - tile_base);
+  // Consecutive threads store consecutive rows of a bin.
+  const int tile_rows = (int)min((int64_t)PT_TILE, a.n - tile_base);
   for (int j = threadIdx.x; j < tile_rows; j += PT_THREADS) {
-    const uint32_t d = digit_of(stage[a.key_col][j], a.mixed, a.shift, a.bits);
-    const int64_t dest = bases[(int64_t)d * a.num_tiles + blockIdx.x] + (j - (int64_t)lstart[d]);
-    for (int c = 0; c < a.ncols; ++c) a.out[c][dest * a.out_stride] = stage[c][j];
+    const int64_t dest = gbase[dig8[j]] + j;
+#pragma unroll
+    for (int c = 0; c < PT_MAXCOLS; ++c)
+      if (c < a.ncols) a.out[c][dest * a.out_stride] = stage[c * PT_TILE + j];
   }
 }
 
@@ -347,7 +373,17 @@ static void run_pass(Context* ctx, PassArgs a) {
                      counts.as<int64_t>());
   QF_CHECK_HIP(hipGetLastError());
   scan_exclusive_i64(ctx, counts.as<int64_t>(), counts.as<int64_t>(), (int64_t)nb * a.num_tiles);
-  hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)a.num_tiles), dim3(PT_THREADS), 0, ctx->stream, a,
+  const size_t lds = scatter_lds_bytes(a.ncols);
+  // Once per device (outside any graph capture in practice: the first pass runs eagerly).
+  static bool lds_attr[64] = {};
+  int dev = 0;
+  QF_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64 || !lds_attr[dev]) {
+    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)scatter_lds_bytes(PT_MAXCOLS)));
+    if (dev >= 0 && dev < 64) lds_attr[dev] = true;
+  }
+  hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)a.num_tiles), dim3(PT_THREADS), lds, ctx->stream, a,
                      counts.as<int64_t>());
   QF_CHECK_HIP(hipGetLastError());
 }
