@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <map>
+#include <type_traits>
 #include <set>
 
 namespace qf {
@@ -163,6 +164,20 @@ __device__ inline uint64_t wave_sum_u64(uint64_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// Sum over the 64 lanes of a wave whose lanes are all active: DPP adds within each row of 16
+// lanes (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror 4, row_ror 8 leave every lane with
+// its row's sum), then the four row sums read out of lanes 0, 16, 32 and 48.
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) + (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
+         (uint32_t)__builtin_amdgcn_readlane((int)v, 32) + (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+__device__ inline uint64_t wave_sum(uint64_t v) { return wave_sum_u64(v); }
+__device__ inline uint64_t wave_sum(uint32_t v) { return wave_sum_u32(v); }
 
 // ------------------------------------------------------------------------------------------
 // HBM tables for heavy partitions, per (relation, join column, subset).
@@ -375,23 +390,26 @@ __device__ inline void load_literal(cptr<DevLit> lits, int li, int nk, int tile,
     }
 }
 
+// Per-lane binding sums: 32 bits for LDS tables (a count is below 2^11 and a run covers at most
+// one tile of rows, so a wave's sum stays below 2^32), 64 bits for HBM tables.
+template <bool kLds>
+using LaneSum = typename std::conditional<kLds, uint32_t, uint64_t>::type;
+
 // Adds a run's counts to literals [l0, l1) (several literals of one group can share a key):
 // covered counts come from ballots (wave-uniform), binding counts from per-lane sums of cnt
-// (positives, negatives) reduced across the wave here.
+// (positives, negatives) reduced across the wave here.  Lanes 0-3 add the four counters.
+template <class Acc>
 __device__ inline void flush_counts(const EvalArgs& a, cptr<DevLit> lits, int l0, int l1, uint64_t& cp,
-                                    uint64_t& cn, uint64_t& lbp, uint64_t& lbn, int lane) {
+                                    uint64_t& cn, Acc& lbp, Acc& lbn, int lane) {
   if (cp | cn) {  // no covered row means no binding either
-    const uint64_t bp = wave_sum_u64(lbp), bn = wave_sum_u64(lbn);
-    if (lane == 0)
-      for (int li = l0; li < l1; ++li) {
-        unsigned long long* out = a.counts_out + (int64_t)lits[li].slot * 4;
-        if (cp) atomicAdd(&out[0], (unsigned long long)cp);
-        if (cn) atomicAdd(&out[1], (unsigned long long)cn);
-        if (bp) atomicAdd(&out[2], (unsigned long long)bp);
-        if (bn) atomicAdd(&out[3], (unsigned long long)bn);
-      }
+    const uint64_t bp = wave_sum(lbp), bn = wave_sum(lbn);
+    const uint64_t v = lane == 0 ? cp : lane == 1 ? cn : lane == 2 ? bp : bn;
+    if (lane < 4 && v != 0)
+      for (int li = l0; li < l1; ++li)
+        atomicAdd(a.counts_out + (int64_t)lits[li].slot * 4 + lane, (unsigned long long)v);
   }
-  cp = cn = lbp = lbn = 0;
+  cp = cn = 0;
+  lbp = lbn = 0;
 }
 
 __device__ inline uint32_t lane_prefix(uint64_t m, int lane) {
@@ -448,7 +466,8 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
   int seg_rows = uni(sh.seg_rows[sg]), seg_off = uni(sh.seg_off[sg]), nblk = (seg_rows + 63) >> 6;
   int g = uni(sh.seg_ug[sg]);
   int jt = U->g[g].jtcol * tile, l0 = U->g[g].sbeg[0], l1 = U->g[g].sbeg[1];
-  uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
+  uint64_t cp = 0, cn = 0;
+  LaneSum<kLds> lbp = 0, lbn = 0;
   for (int e = e0; e < e1;) {
     const bool two = e + 1 < e1 && blk + 1 < nblk;
     const int ra = blk * 64 + lane, rb = ra + 64;
@@ -539,7 +558,8 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
   int lit0 = U->g[uni(sh.seg_ug[sg])].sbeg[s];
   LitRegs L;
   load_literal<NK>(lits, lit0 + lit, nk, tile, L);
-  uint64_t cp = 0, cn = 0, lbp = 0, lbn = 0;
+  uint64_t cp = 0, cn = 0;
+  LaneSum<kLds> lbp = 0, lbn = 0;
   for (int e = e0; e < e1; ++e) {
     const int rloc = blk * 64 + lane;
     uint64_t pm = ~0ull;
