@@ -915,13 +915,32 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
   }
 }
 
-// Sums the blocks' counter copies: out[w] = sum over c of copies[c * words + w].
-__global__ void k_sum_copies(const uint64_t* copies, int ncopies, int64_t words, uint64_t* out) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= words) return;
-  uint64_t sum = 0;
-  for (int c = 0; c < ncopies; ++c) sum += copies[(int64_t)c * words + w];
-  out[w] = sum;
+// Sums the blocks' counter copies: out[w] = sum over c of copies[c * words + w].  A block takes
+// 64 words; its 16 waves split the copies (loads coalesced over the words), then add in LDS.
+constexpr int SUM_WORDS = 64, SUM_GROUPS = 16;
+__global__ __launch_bounds__(SUM_WORDS * SUM_GROUPS) void k_sum_copies(const uint64_t* copies, int ncopies,
+                                                                     int64_t words, uint64_t* out) {
+  __shared__ uint64_t part[SUM_GROUPS][SUM_WORDS];
+  const int x = threadIdx.x & (SUM_WORDS - 1), g = threadIdx.x / SUM_WORDS;
+  const int64_t w = (int64_t)blockIdx.x * SUM_WORDS + x;
+  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  if (w < words) {
+    int c = g;
+    for (; c + 3 * SUM_GROUPS < ncopies; c += 4 * SUM_GROUPS) {
+      s0 += copies[(int64_t)c * words + w];
+      s1 += copies[(int64_t)(c + SUM_GROUPS) * words + w];
+      s2 += copies[(int64_t)(c + 2 * SUM_GROUPS) * words + w];
+      s3 += copies[(int64_t)(c + 3 * SUM_GROUPS) * words + w];
+    }
+    for (; c < ncopies; c += SUM_GROUPS) s0 += copies[(int64_t)c * words + w];
+  }
+  part[g][x] = s0 + s1 + s2 + s3;
+  __syncthreads();
+  if (g == 0 && w < words) {
+    uint64_t sum = 0;
+    for (int k = 0; k < SUM_GROUPS; ++k) sum += part[k][x];
+    out[w] = sum;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1509,7 +1528,8 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
   }
   if (P.num_slots > 0) {
     const int64_t words = (int64_t)P.num_slots * 4;
-    hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)ceil_div(words, 256)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)ceil_div(words, SUM_WORDS)), dim3(SUM_WORDS * SUM_GROUPS), 0,
+                       ctx->stream,
                        P.d_copies.as<uint64_t>(), P.ncopies, words, P.d_counts.as<uint64_t>());
     QF_CHECK_HIP(hipGetLastError());
   }

@@ -242,6 +242,14 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_hist(PassArgs a, int64_t* c
   for (int b = threadIdx.x; b < nb; b += PT_THREADS) counts[(int64_t)b * a.num_tiles + blockIdx.x] = hist[b];
 }
 
+// Tile of workgroup b when workgroup b runs on XCD b % 8: XCD x takes tiles [x*n/8, (x+1)*n/8).
+__device__ inline int64_t xcd_tile(int64_t b, int64_t n) {
+  const int64_t x = b & 7, i = b >> 3;
+  const int64_t q = n >> 3, r = n & 7;  // XCDs below r take q + 1 tiles
+  const int64_t start = x * q + (x < r ? x : r);
+  return start + i;
+}
+
 // LDS of the scatter kernel beyond its static arrays: the staged tile (ncols x PT_TILE words)
 // and one digit byte per row.
 static size_t scatter_lds_bytes(int ncols) { return (size_t)ncols * PT_TILE * 4 + PT_TILE; }
@@ -260,8 +268,11 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
   const int w = threadIdx.x >> 6;
   for (int b = threadIdx.x; b < PT_WAVES * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0])[b] = 0;
 
+  // Workgroups are dealt to the 8 XCDs round robin; give each XCD one contiguous range of tiles,
+  // so that the runs neighbouring tiles write into one bin meet in the same L2.
+  const int64_t tile = xcd_tile(blockIdx.x, a.num_tiles);
   // Each wave owns PT_IPT consecutive 64-row groups of the tile, so (wave, group, lane) is row order.
-  const int64_t tile_base = (int64_t)blockIdx.x * PT_TILE;
+  const int64_t tile_base = tile * PT_TILE;
   const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
   const int32_t* key = a.in[a.key_col];
   int32_t v[PT_IPT][PT_MAXCOLS];
@@ -303,7 +314,7 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(6)))
       wrun[ww][threadIdx.x] = run;
       run += c;
     }
-    gbase[threadIdx.x] = bases[(int64_t)threadIdx.x * a.num_tiles + blockIdx.x] - (int64_t)excl;
+    gbase[threadIdx.x] = bases[(int64_t)threadIdx.x * a.num_tiles + tile] - (int64_t)excl;
   }
   __syncthreads();
 
