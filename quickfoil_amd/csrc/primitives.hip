@@ -222,180 +222,273 @@ __device__ inline uint32_t digit_of(int32_t k, int mixed, int shift, int bits) {
   return bits == 0 ? 0u : (h >> shift) & ((1u << bits) - 1u);
 }
 
+// Digit counts of one tile per wave (eight tiles per workgroup), written bin-major
+// (counts[b * num_tiles + t]) so that one exclusive scan gives every tile its output base per bin.
+constexpr int PT_HIST_RPL = PT_TILE / 64;  // rows per lane
 __global__ __launch_bounds__(PT_THREADS) void k_part_hist(PassArgs a, int64_t* counts) {
-  __shared__ uint32_t hist[PT_MAXBINS];
+  __shared__ uint32_t hist[PT_WAVES][PT_MAXBINS];
   const int nb = 1 << a.bits;
-  for (int b = threadIdx.x; b < nb; b += PT_THREADS) hist[b] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * PT_TILE;
-  const int32_t* key = a.in[a.key_col];
-  uint32_t d[PT_IPT];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t tile = (int64_t)blockIdx.x * PT_WAVES + w;
+  if (tile >= a.num_tiles) return;  // whole waves only; no workgroup barrier follows
+  for (int b = lane; b < nb; b += 64) hist[w][b] = 0;
+  __builtin_amdgcn_wave_barrier();
+  const int64_t base = tile * PT_TILE;
+  const int32_t* key = a.in[a.key_col] + base;
+  const int rows = (int)min((int64_t)PT_TILE, a.n - base);
+  int32_t k[PT_HIST_RPL];
+  if (rows == PT_TILE) {
 #pragma unroll
-  for (int i = 0; i < PT_IPT; ++i) {
-    const int64_t row = base + (int64_t)i * PT_THREADS + threadIdx.x;
-    d[i] = row < a.n ? digit_of(key[row], a.mixed, a.shift, a.bits) : PT_MAXBINS;
+    for (int i = 0; i < PT_HIST_RPL; ++i) k[i] = key[i * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < PT_HIST_RPL; ++i) atomicAdd(&hist[w][digit_of(k[i], a.mixed, a.shift, a.bits)], 1u);
+  } else {
+    for (int r = lane; r < rows; r += 64) atomicAdd(&hist[w][digit_of(key[r], a.mixed, a.shift, a.bits)], 1u);
   }
-#pragma unroll
-  for (int i = 0; i < PT_IPT; ++i)
-    if (d[i] < PT_MAXBINS) atomicAdd(&hist[d[i]], 1u);
-  __syncthreads();
-  for (int b = threadIdx.x; b < nb; b += PT_THREADS) counts[(int64_t)b * a.num_tiles + blockIdx.x] = hist[b];
-}
-
-// Tile of workgroup b when workgroup b runs on XCD b % 8: XCD x takes tiles [x*n/8, (x+1)*n/8).
-__device__ inline int64_t xcd_tile(int64_t b, int64_t n) {
-  const int64_t x = b & 7, i = b >> 3;
-  const int64_t q = n >> 3, r = n & 7;  // XCDs below r take q + 1 tiles
-  const int64_t start = x * q + (x < r ? x : r);
-  return start + i;
+  __builtin_amdgcn_wave_barrier();
+  for (int b = lane; b < nb; b += 64) counts[(int64_t)b * a.num_tiles + tile] = hist[w][b];
 }
 
 // LDS of the scatter kernel beyond its static arrays: the staged tile (ncols x PT_TILE words)
 // and one digit byte per row.
 static size_t scatter_lds_bytes(int ncols) { return (size_t)ncols * PT_TILE * 4 + PT_TILE; }
 
-// 6 waves per SIMD (at most 80 VGPRs): three blocks per CU, as their LDS allows for up to 5 columns.
+// Tiles of one scatter workgroup.  Workgroups are dealt to the 8 XCDs round robin; when the grid
+// is a multiple of 8, XCD x owns the contiguous tiles [x*n/8, (x+1)*n/8) and its workgroups
+// stride through them, so the tiles an XCD works on at once are neighbours and the runs they
+// write into one bin meet in that XCD's L2.
+struct TileWalk {
+  int64_t next, end, step;
+  __device__ TileWalk(int64_t n) {
+    const int64_t G = gridDim.x, b = blockIdx.x;
+    if (G % 8 == 0) {
+      const int64_t x = b & 7, q = n >> 3, r = n & 7;
+      const int64_t start = x * q + (x < r ? x : r);
+      next = start + (b >> 3);
+      end = start + q + (x < r ? 1 : 0);
+      step = G >> 3;
+    } else {
+      next = b;
+      end = n;
+      step = G;
+    }
+  }
+};
+
+// Loads one tile's rows (NC columns) into registers: every column's loads are in flight together.
+// Addresses are a wave-uniform base per column plus the lane's offset.
+template <int NC>
+__device__ inline void load_tile(const PassArgs& a, int64_t tile, int lane, int w, int32_t (&v)[PT_IPT][NC]) {
+  const int64_t wave_base = tile * PT_TILE + (int64_t)w * 64 * PT_IPT;
+  if (wave_base + 64 * PT_IPT <= a.n) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c == a.tid_col) {
+#pragma unroll
+        for (int i = 0; i < PT_IPT; ++i) v[i][c] = (int32_t)(wave_base + i * 64 + lane);
+      } else {
+        const int32_t* src = a.in[c] + wave_base;
+#pragma unroll
+        for (int i = 0; i < PT_IPT; ++i) v[i][c] = src[i * 64 + lane];
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < PT_IPT; ++i) {
+    const int64_t row = wave_base + (int64_t)i * 64 + lane;
+    const bool ok = row < a.n;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[i][c] = c == a.tid_col ? (int32_t)row : (ok ? a.in[c][row] : 0);
+  }
+}
+
+// Persistent workgroups, software pipelined: a tile's rows are loaded into registers while the
+// previous tile is written out of LDS.
+// NC = a.ncols.  6 waves per SIMD (at most 80 VGPRs): three blocks per CU, as their LDS allows
+// for up to 5 columns.
+template <int NC>
 __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_part_scatter(
     PassArgs a, const int64_t* bases) {
   extern __shared__ int32_t stage[];                // ncols x PT_TILE rows in bin order, then digits
-  __shared__ uint32_t wrun[PT_WAVES][PT_MAXBINS];  // per-wave counts, then per-wave running starts
-  __shared__ int64_t gbase[PT_MAXBINS];            // output row of bin b's first tile row, minus its tile start
+  // Per-wave counts, then per-wave running starts, 16 bits each (a tile has 2048 rows): wave w
+  // uses half (w & 1) of row (w >> 1).
+  __shared__ uint32_t wrun[PT_WAVES / 2][PT_MAXBINS];
+  __shared__ int64_t gbase[PT_MAXBINS];  // output row of bin b's first tile row, minus its tile start
   __shared__ uint32_t wtot[PT_MAXBINS / 64];
   uint8_t* dig8 = reinterpret_cast<uint8_t*>(stage + a.ncols * PT_TILE);
 
   const int nb = 1 << a.bits;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  for (int b = threadIdx.x; b < PT_WAVES * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0])[b] = 0;
-
-  // Workgroups are dealt to the 8 XCDs round robin; give each XCD one contiguous range of tiles,
-  // so that the runs neighbouring tiles write into one bin meet in the same L2.
-  const int64_t tile = xcd_tile(blockIdx.x, a.num_tiles);
-  // Each wave owns PT_IPT consecutive 64-row groups of the tile, so (wave, group, lane) is row order.
-  const int64_t tile_base = tile * PT_TILE;
-  const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
-  const int32_t* key = a.in[a.key_col];
-  int32_t v[PT_IPT][PT_MAXCOLS];
-  uint32_t d[PT_IPT];
-#pragma unroll
-  for (int i = 0; i < PT_IPT; ++i) {
-    const int64_t row = wave_base + (int64_t)i * 64 + lane;
-    const bool ok = row < a.n;
-    d[i] = ok ? digit_of(key[row], a.mixed, a.shift, a.bits) : 0u;
-#pragma unroll
-    for (int c = 0; c < PT_MAXCOLS; ++c)
-      if (c < a.ncols) v[i][c] = c == a.tid_col ? (int32_t)row : (ok ? a.in[c][row] : 0);
-  }
-  __syncthreads();  // counts cleared
-#pragma unroll
-  for (int i = 0; i < PT_IPT; ++i)
-    if (wave_base + (int64_t)i * 64 + lane < a.n) atomicAdd(&wrun[w][d[i]], 1u);
-  __syncthreads();
-
-  // Bin totals, their exclusive scan (one lane per bin, 64 bins per wave), per-wave starts.
-  uint32_t tot = 0, incl = 0;
-  if (threadIdx.x < PT_MAXBINS) {
-    if ((int)threadIdx.x < nb)
-      for (int ww = 0; ww < PT_WAVES; ++ww) tot += wrun[ww][threadIdx.x];
-    incl = tot;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += y;
-    }
-    if (lane == 63) wtot[w] = incl;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < nb) {
-    uint32_t excl = incl - tot;
-    for (int ww = 0; ww < w; ++ww) excl += wtot[ww];
-    uint32_t run = excl;
-    for (int ww = 0; ww < PT_WAVES; ++ww) {
-      const uint32_t c = wrun[ww][threadIdx.x];
-      wrun[ww][threadIdx.x] = run;
-      run += c;
-    }
-    gbase[threadIdx.x] = bases[(int64_t)threadIdx.x * a.num_tiles + tile] - (int64_t)excl;
-  }
-  __syncthreads();
-
-  // Stable ranks within the wave (peers: lanes of the group with the same digit, one ballot per
-  // digit bit), rows staged in LDS in bin order.
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-  for (int i = 0; i < PT_IPT; ++i) {
-    const int64_t row = wave_base + (int64_t)i * 64 + lane;
-    const bool valid = row < a.n;
-    const uint64_t vmask = __ballot(valid);
-    if (vmask == 0) break;
-    uint64_t peers = vmask;
-    for (int k = 0; k < a.bits; ++k) {
-      const bool bit = (d[i] >> k) & 1u;
-      const uint64_t bk = __ballot(bit);
-      peers &= bit ? bk : ~bk;
-    }
-    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    uint32_t pos = 0;
-    if (valid) pos = wrun[w][d[i]] + rank;
-    __builtin_amdgcn_wave_barrier();
-    if (valid) {
-#pragma unroll
-      for (int c = 0; c < PT_MAXCOLS; ++c)
-        if (c < a.ncols) stage[c * PT_TILE + pos] = v[i][c];
-      dig8[pos] = (uint8_t)d[i];
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) wrun[w][d[i]] += (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-  }
-  __syncthreads();
+  const int wsh = 16 * (w & 1);
+  TileWalk walk(a.num_tiles);
+  int32_t v[PT_IPT][NC];
+  if (walk.next < walk.end) load_tile<NC>(a, walk.next, lane, w, v);
 
-  // Consecutive threads store consecutive rows of a bin.
-  const int tile_rows = (int)min((int64_t)PT_TILE, a.n - tile_base);
-  for (int j = threadIdx.x; j < tile_rows; j += PT_THREADS) {
-    const int64_t dest = gbase[dig8[j]] + j;
+  for (int64_t tile = walk.next; tile < walk.end; tile += walk.step) {
+    // Each wave owns PT_IPT consecutive 64-row groups of the tile: (wave, group, lane) is row order.
+    const int64_t tile_base = tile * PT_TILE;
+    const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
+    for (int b = threadIdx.x; b < PT_WAVES / 2 * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0])[b] = 0;
+    uint32_t d[PT_IPT];
 #pragma unroll
-    for (int c = 0; c < PT_MAXCOLS; ++c)
-      if (c < a.ncols) a.out[c][dest * a.out_stride] = stage[c * PT_TILE + j];
+    for (int i = 0; i < PT_IPT; ++i) {
+      int32_t key = v[i][0];
+#pragma unroll
+      for (int c = 1; c < NC; ++c)
+        if (c == a.key_col) key = v[i][c];
+      d[i] = digit_of(key, a.mixed, a.shift, a.bits);
+    }
+    __syncthreads();  // counts cleared; the previous tile's write-out is done with LDS
+#pragma unroll
+    for (int i = 0; i < PT_IPT; ++i)
+      if (wave_base + (int64_t)i * 64 + lane < a.n) atomicAdd(&wrun[w >> 1][d[i]], 1u << wsh);
+    __syncthreads();
+
+    // Bin totals, their exclusive scan (one lane per bin, 64 bins per wave), per-wave starts.
+    uint32_t tot = 0, incl = 0;
+    if (threadIdx.x < PT_MAXBINS) {
+      if ((int)threadIdx.x < nb)
+        for (int ww = 0; ww < PT_WAVES / 2; ++ww) {
+          const uint32_t pair = wrun[ww][threadIdx.x];
+          tot += (pair & 0xffffu) + (pair >> 16);
+        }
+      incl = tot;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+      }
+      if (lane == 63) wtot[w] = incl;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nb) {
+      uint32_t excl = incl - tot;
+      for (int ww = 0; ww < w; ++ww) excl += wtot[ww];
+      uint32_t at = excl;
+      for (int ww = 0; ww < PT_WAVES / 2; ++ww) {
+        const uint32_t pair = wrun[ww][threadIdx.x];
+        const uint32_t lo = at, hi = at + (pair & 0xffffu);
+        wrun[ww][threadIdx.x] = lo | (hi << 16);
+        at = hi + (pair >> 16);
+      }
+      gbase[threadIdx.x] = bases[(int64_t)threadIdx.x * a.num_tiles + tile] - (int64_t)excl;
+    }
+    __syncthreads();
+
+    // Stable ranks within the wave (peers: lanes of the group with the same digit, one ballot per
+    // digit bit), rows staged in LDS in bin order.
+#pragma unroll
+    for (int i = 0; i < PT_IPT; ++i) {
+      const int64_t row = wave_base + (int64_t)i * 64 + lane;
+      const bool valid = row < a.n;
+      const uint64_t vmask = __ballot(valid);
+      if (vmask == 0) break;
+      uint64_t peers = vmask;
+      for (int k = 0; k < a.bits; ++k) {
+        const bool bit = (d[i] >> k) & 1u;
+        const uint64_t bk = __ballot(bit);
+        peers &= bit ? bk : ~bk;
+      }
+      const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+      uint32_t pos = 0;
+      if (valid) pos = ((wrun[w >> 1][d[i]] >> wsh) & 0xffffu) + rank;
+      __builtin_amdgcn_wave_barrier();
+      if (valid) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) stage[c * PT_TILE + pos] = v[i][c];
+        dig8[pos] = (uint8_t)d[i];
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (valid && rank == 0) atomicAdd(&wrun[w >> 1][d[i]], (uint32_t)__popcll(peers) << wsh);
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // The next tile's loads fly while this one is written out.
+    if (tile + walk.step < walk.end) load_tile<NC>(a, tile + walk.step, lane, w, v);
+
+    // Consecutive threads store consecutive rows of a bin.
+    const int tile_rows = (int)min((int64_t)PT_TILE, a.n - tile_base);
+    for (int j = threadIdx.x; j < tile_rows; j += PT_THREADS) {
+      const int64_t dest = gbase[dig8[j]] + j;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a.out[c][dest * a.out_stride] = stage[c * PT_TILE + j];
+    }
   }
 }
 
 // offsets[b] = first row whose digit is >= b, for b in [0, num_parts]; rows are sorted by digit.
+// Each thread takes PT_OFF_RPT rows, 256 apart (coalesced).
+constexpr int PT_OFF_RPT = 8;
 __global__ void k_part_offsets(const int32_t* key, int stride, int64_t n, int mixed, int shift, int bits,
                                int64_t* offsets, int64_t num_parts) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x * PT_OFF_RPT + threadIdx.x;
   if (n == 0) {
-    if (i == 0)
+    if (i0 == 0)
       for (int64_t b = 0; b <= num_parts; ++b) offsets[b] = 0;
     return;
   }
-  if (i >= n) return;
-  const int64_t d = digit_of(key[i * stride], mixed, shift, bits);
-  const int64_t dp = i == 0 ? -1 : (int64_t)digit_of(key[(i - 1) * stride], mixed, shift, bits);
-  for (int64_t b = dp + 1; b <= d; ++b) offsets[b] = i;
-  if (i == n - 1)
-    for (int64_t b = d + 1; b <= num_parts; ++b) offsets[b] = n;
+  // All keys first (the stores below could alias them as far as the compiler knows).
+  int32_t k[PT_OFF_RPT], kp[PT_OFF_RPT];
+#pragma unroll
+  for (int j = 0; j < PT_OFF_RPT; ++j) {
+    const int64_t i = i0 + (int64_t)j * blockDim.x;
+    k[j] = i < n ? key[i * stride] : 0;
+    kp[j] = (i < n && i > 0) ? key[(i - 1) * stride] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < PT_OFF_RPT; ++j) {
+    const int64_t i = i0 + (int64_t)j * blockDim.x;
+    if (i >= n) return;
+    const int64_t d = digit_of(k[j], mixed, shift, bits);
+    const int64_t dp = i == 0 ? -1 : (int64_t)digit_of(kp[j], mixed, shift, bits);
+    for (int64_t b = dp + 1; b <= d; ++b) offsets[b] = i;
+    if (i == n - 1)
+      for (int64_t b = d + 1; b <= num_parts; ++b) offsets[b] = n;
+  }
+}
+
+template <int NC>
+static void launch_scatter(Context* ctx, const PassArgs& a, const int64_t* bases) {
+  const size_t lds = scatter_lds_bytes(NC);
+  // Once per device (outside any graph capture in practice: the first pass runs eagerly).
+  static bool lds_attr[64] = {};
+  int dev = 0;
+  QF_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64 || !lds_attr[dev]) {
+    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter<NC>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (dev >= 0 && dev < 64) lds_attr[dev] = true;
+  }
+  // Persistent grid: as many workgroups as fit the GPU at once (a multiple of 8), at most one per tile.
+  int per_cu = 0;
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter<NC>, PT_THREADS, lds));
+  int64_t grid = (int64_t)std::max(1, per_cu) * ctx->num_cus;
+  if (grid >= 8) grid &= ~int64_t(7);
+  grid = std::min<int64_t>(grid, a.num_tiles);
+  hipLaunchKernelGGL(k_part_scatter<NC>, dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
 }
 
 static void run_pass(Context* ctx, PassArgs a) {
   a.num_tiles = std::max<int64_t>(1, ceil_div(a.n, PT_TILE));
   const int nb = 1 << a.bits;
   DevBuf counts(ctx, (size_t)nb * a.num_tiles * sizeof(int64_t));
-  hipLaunchKernelGGL(k_part_hist, dim3((unsigned)a.num_tiles), dim3(PT_THREADS), 0, ctx->stream, a,
-                     counts.as<int64_t>());
+  hipLaunchKernelGGL(k_part_hist, dim3((unsigned)ceil_div(a.num_tiles, PT_WAVES)), dim3(PT_THREADS), 0,
+                     ctx->stream, a, counts.as<int64_t>());
   QF_CHECK_HIP(hipGetLastError());
   scan_exclusive_i64(ctx, counts.as<int64_t>(), counts.as<int64_t>(), (int64_t)nb * a.num_tiles);
-  const size_t lds = scatter_lds_bytes(a.ncols);
-  // Once per device (outside any graph capture in practice: the first pass runs eagerly).
-  static bool lds_attr[64] = {};
-  int dev = 0;
-  QF_CHECK_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64 || !lds_attr[dev]) {
-    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)scatter_lds_bytes(PT_MAXCOLS)));
-    if (dev >= 0 && dev < 64) lds_attr[dev] = true;
+  switch (a.ncols) {
+    case 1: launch_scatter<1>(ctx, a, counts.as<int64_t>()); break;
+    case 2: launch_scatter<2>(ctx, a, counts.as<int64_t>()); break;
+    case 3: launch_scatter<3>(ctx, a, counts.as<int64_t>()); break;
+    case 4: launch_scatter<4>(ctx, a, counts.as<int64_t>()); break;
+    case 5: launch_scatter<5>(ctx, a, counts.as<int64_t>()); break;
+    case 6: launch_scatter<6>(ctx, a, counts.as<int64_t>()); break;
+    case 7: launch_scatter<7>(ctx, a, counts.as<int64_t>()); break;
+    default: launch_scatter<8>(ctx, a, counts.as<int64_t>()); break;
   }
-  hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)a.num_tiles), dim3(PT_THREADS), lds, ctx->stream, a,
-                     counts.as<int64_t>());
   QF_CHECK_HIP(hipGetLastError());
 }
 
@@ -447,7 +540,7 @@ static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_
   if (d_offsets) {
     const int64_t parts = (int64_t)1 << total_bits;
     const int full_shift = mixed ? 32 - total_bits : 0;
-    const int64_t blocks = std::max<int64_t>(1, ceil_div(num_rows, 256));
+    const int64_t blocks = std::max<int64_t>(1, ceil_div(num_rows, 256 * PT_OFF_RPT));
     hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, out_cols[key_col],
                        out_stride, num_rows, mixed, total_bits == 0 ? 0 : full_shift, total_bits, d_offsets,
                        parts);
