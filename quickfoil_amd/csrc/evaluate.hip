@@ -702,15 +702,14 @@ struct FactPrefetch {
   int64_t q = -1;
 };
 
+// Register j of thread t carries fact row t of column j (columns below EV_PF, rows below
+// EV_THREADS); the other words are loaded when the unit starts.
 __device__ inline void prefetch_facts(cptr<DevUnit> U, int unit, int64_t q, int64_t p0, int np, FactPrefetch& pf) {
-  const int words = np * U->arity;
+  const int arity = U->arity;
+  if ((int)threadIdx.x < np) {
 #pragma unroll
-  for (int j = 0; j < EV_PF; ++j) {
-    const int f = threadIdx.x + EV_THREADS * j;
-    if (f < words) {
-      const int c = f / np, i = f - c * np;
-      pf.v[j] = U->pcols[c][p0 + i];
-    }
+    for (int j = 0; j < EV_PF; ++j)
+      if (j < arity) pf.v[j] = U->pcols[j][p0 + threadIdx.x];
   }
   pf.unit = unit;
   pf.q = q;
@@ -831,15 +830,15 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
     if (hq < 0) {
       const int unit = B->unit[k];
       if (pf.unit == unit && pf.q == q) {
-        const int words = np * U->arity;
+        const int arity = U->arity;
+        if ((int)threadIdx.x < np) {
 #pragma unroll
-        for (int j = 0; j < EV_PF; ++j) {
-          const int f = threadIdx.x + EV_THREADS * j;
-          if (f < words) pstage[f] = pf.v[j];
+          for (int j = 0; j < EV_PF; ++j)
+            if (j < arity) pstage[j * np + threadIdx.x] = pf.v[j];
         }
-        for (int f = threadIdx.x + EV_THREADS * EV_PF; f < words; f += EV_THREADS) {
-          const int c = f / np, i = f - c * np;
-          pstage[f] = U->pcols[c][p0 + i];
+        for (int c = 0; c < arity; ++c) {
+          const int32_t* src = U->pcols[c] + p0;
+          for (int i = threadIdx.x + (c < EV_PF ? EV_THREADS : 0); i < np; i += EV_THREADS) pstage[c * np + i] = src[i];
         }
       } else {
         for (int c = 0; c < U->arity; ++c) {
