@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="fixed 512 facts per partition and 1024-row tiles instead of the autotuner's choice")
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB of a device copy to run once")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -142,10 +144,17 @@ def main():
     ctx.enable_timing(True)
     L = len(w["literals"])
 
+    # The batched form takes its facts per partition and tile from the autotuner (it times a grid
+    # of both on the first call for this workload shape); the plain form reuses the choice.
+    tune = {"radix_bits": args.radix_bits, "tile_rows": args.tile_rows, "partition_rows": 0}
+
     def step(mode):
+        auto = mode == "batched" and not args.no_autotune and not args.radix_bits and not args.tile_rows
         counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], w["literals"], mode=mode,
-                                          reuse_partitions=True, radix_bits=args.radix_bits, tile_rows=args.tile_rows,
-                                          use_graph=(mode == "batched" and not args.no_graph))
+                                          reuse_partitions=True, radix_bits=tune["radix_bits"],
+                                          tile_rows=0 if auto else tune["tile_rows"],
+                                          partition_rows=0 if auto else tune["partition_rows"],
+                                          use_graph=(mode == "batched" and not args.no_graph), autotune=auto)
         return counts
 
     # Warm-up (first step partitions and caches the relations: the cold cost).
@@ -153,7 +162,12 @@ def main():
     ref_counts = step("batched")
     torch.cuda.synchronize()
     cold_s = time.perf_counter() - tc
-    log("cold batched step: %.3f s" % cold_s)
+    log("cold batched step (with autotuning): %.3f s" % cold_s)
+    if not args.no_autotune and not args.radix_bits and not args.tile_rows:
+        choice = ctx.autotune_choice()
+        pr, tr, best_ms = choice["partition_rows"], choice["tile_rows"], choice["best_ms"]
+        tune.update(partition_rows=pr, tile_rows=tr)
+        log("autotuner: %d facts per partition, %d-row tiles (%.2f ms)" % (pr, tr, best_ms))
     for _ in range(max(0, args.warmup - 1)):
         step("batched")
     torch.cuda.synchronize()
@@ -173,8 +187,8 @@ def main():
     ctx.reset_timing()
     for i in range(args.steps):
         eager_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], w["literals"],
-                                                mode="batched", reuse_partitions=True, radix_bits=args.radix_bits,
-                                                tile_rows=args.tile_rows)
+                                                mode="batched", reuse_partitions=True, radix_bits=tune["radix_bits"],
+                                                tile_rows=tune["tile_rows"], partition_rows=tune["partition_rows"])
     torch.cuda.synchronize()
     assert np.array_equal(eager_counts, ref_counts), "eager and graph counts differ"
     phases = ctx.phase_times()
@@ -229,7 +243,8 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded generators)",
         "config": {"workload": workload_name, "state": "warm (relation partitions cached)",
-                   "literals": L, "join_groups": num_groups},
+                   "literals": L, "join_groups": num_groups,
+                   "facts_per_partition": tune["partition_rows"] or 512, "tile_rows": tune["tile_rows"] or 1024},
         "plain_literals_per_s": (L / plain_s) if plain_s else None,
         "plain_ms_per_step": (plain_s * 1e3) if plain_s else None,
         "batched_over_plain": (plain_s / batched_s) if plain_s else None,
