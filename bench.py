@@ -64,6 +64,18 @@ def algorithmic_probe_bytes(workload):
     return total, len(groups)
 
 
+def algorithmic_partition_bytes(workload):
+    """Per step: the binding rows are partitioned once per clause join variable, reading every
+    referenced binding column once and writing it plus the row id once (4 B each)."""
+    nb = int(workload["binding"][0].numel())
+    join_vars, cols = set(), set()
+    for _, vars_ in workload["literals"]:
+        bound = [v for v in vars_ if v >= 0]
+        join_vars.add(bound[0])
+        cols.update(bound)
+    return len(join_vars) * nb * 4 * (2 * len(cols) + 1)
+
+
 def measured_traffic(workload):
     """HBM bytes per k_eval launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     this workload (profiles/k_eval_traffic.json), or None when none was recorded."""
@@ -216,6 +228,9 @@ def main():
         log("copy calibration: %d MiB" % args.copy_calibration)
 
     algo_bytes, num_groups = algorithmic_probe_bytes(w)
+    part_bytes = algorithmic_partition_bytes(w)
+    part_ms_per_step = phases.get("partition_binding", (0.0, 0))[0] / max(1, args.steps)
+    part_achieved = part_bytes / (part_ms_per_step * 1e-3) / 1e9 if part_ms_per_step > 0 else 0.0
     probe_ms, probe_launches = phases.get("probe_count", (0.0, 0))
     probe_ms_per_step = probe_ms / max(1, args.steps)
     achieved = algo_bytes / (probe_ms_per_step * 1e-3) / 1e9 if probe_ms_per_step > 0 else 0.0
@@ -254,6 +269,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_eval (probe and count; HIP events around each launch on the context stream)",
                      "algorithmic_bytes_per_step": algo_bytes},
+        "roofline_partition": {"bound": "hbm", "achieved": part_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": part_achieved / HBM_PEAK_GBS, "traffic": None,
+                               "kernel": "binding radix partition per join variable (histogram, scan, scatter, "
+                                         "offsets; HIP events around the phase)",
+                               "algorithmic_bytes_per_step": part_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
