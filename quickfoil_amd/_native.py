@@ -380,7 +380,9 @@ class Context:
                                                                  num_binding_pos, num_binding_neg, arr))
         return [o[:rows] for o in outs]
 
-    def create_binding_table(self, binding_columns, relation_columns, vars_):
+    def create_binding_table(self, binding_columns, relation_columns, vars_, out=None):
+        """CreateBindingTable in two calls: the row count, then the rows (into `out`, a list of
+        int32 device buffers, one per output column, when they are long enough)."""
         t = self.torch
         b = _TableHolder(binding_columns)
         r = _TableHolder(relation_columns)
@@ -390,22 +392,31 @@ class Context:
                                                      None, ctypes.byref(rows)))
         n_new = sum(1 for v in vars_ if v < 0 or v >= len(binding_columns))
         dev = binding_columns[0].device
-        outs = [t.empty(max(1, rows.value), dtype=t.int32, device=dev) for _ in range(len(binding_columns) + n_new)]
+        ncols = len(binding_columns) + n_new
+        if out is not None and len(out) >= ncols and all(o.numel() >= max(1, rows.value) for o in out[:ncols]):
+            outs = list(out[:ncols])
+        else:
+            outs = [t.empty(max(1, rows.value), dtype=t.int32, device=dev) for _ in range(ncols)]
         arr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
         self._check(self.lib.qf_create_binding_table(self.h, ctypes.byref(b.t), ctypes.byref(r.t), ctypes.byref(lit),
                                                      arr, ctypes.byref(rows)))
         return [o[: rows.value] for o in outs]
 
-    def create_binding_table_presized(self, binding_columns, relation_columns, vars_, capacity):
-        """CreateBindingTable into buffers of `capacity` rows; on overflow the exact size is used
-        for a second call.  Returns (columns, fell_back)."""
+    def create_binding_table_presized(self, binding_columns, relation_columns, vars_, capacity, out=None):
+        """CreateBindingTable into buffers of `capacity` rows (the first `capacity` rows of `out`
+        when given); on overflow the exact size is used for a second call.  Returns (columns,
+        fell_back)."""
         t = self.torch
         b = _TableHolder(binding_columns)
         r = _TableHolder(relation_columns)
         lit = self._literal(0, vars_)
         n_new = sum(1 for v in vars_ if v < 0 or v >= len(binding_columns))
         dev = binding_columns[0].device
-        outs = [t.empty(max(1, capacity), dtype=t.int32, device=dev) for _ in range(len(binding_columns) + n_new)]
+        ncols = len(binding_columns) + n_new
+        if out is not None and len(out) >= ncols and all(o.numel() >= max(1, capacity) for o in out[:ncols]):
+            outs = list(out[:ncols])
+        else:
+            outs = [t.empty(max(1, capacity), dtype=t.int32, device=dev) for _ in range(ncols)]
         arr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
         rows = ctypes.c_int64(0)
         st = self.lib.qf_create_binding_table_presized(self.h, ctypes.byref(b.t), ctypes.byref(r.t),
@@ -414,7 +425,8 @@ class Context:
             return [o[: rows.value] for o in outs], False
         if st != 4:  # anything but the overflow that asks for the exact size
             self._check(st)
-        outs = [t.empty(max(1, rows.value), dtype=t.int32, device=dev) for _ in range(len(binding_columns) + n_new)]
+        if not (out is not None and len(out) >= ncols and all(o.numel() >= max(1, rows.value) for o in out[:ncols])):
+            outs = [t.empty(max(1, rows.value), dtype=t.int32, device=dev) for _ in range(ncols)]
         arr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
         self._check(self.lib.qf_create_binding_table(self.h, ctypes.byref(b.t), ctypes.byref(r.t), ctypes.byref(lit),
                                                      arr, ctypes.byref(rows)))

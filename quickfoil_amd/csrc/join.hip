@@ -279,7 +279,10 @@ int qf_create_label_aware_binding_table(qf_context* c, const qf_table* binding, 
 static int create_binding_table(qf_context* c, const qf_table* binding, const qf_table* relation,
                                 const qf_literal* literal, int32_t* const* d_out_columns, int64_t* h_num_rows,
                                 int64_t capacity) {
-  return guarded(c, [&](Context* ctx) {
+  // A presized call whose rows exceed the capacity is an expected outcome (the caller retries
+  // with the exact size): it returns QF_ERR_OVERFLOW without going through an exception.
+  bool over_capacity = false;
+  const int st = guarded(c, [&](Context* ctx) {
     check_tab(binding, "binding table");
     check_tab(relation, "relation");
     QF_REQUIRE(literal && h_num_rows, "null literal or row count");
@@ -318,8 +321,13 @@ static int create_binding_table(qf_context* c, const qf_table* binding, const qf
     else total = ordered_join(ctx, lit_keys, nbg, bind_keys, N, p, write, expect, capacity);
     *h_num_rows = total;
     if (write && capacity < 0 && total != expect) throw Error(QF_ERR_OVERFLOW, "output size differs from the row count");
-    if (write && capacity >= 0 && total > capacity) throw Error(QF_ERR_OVERFLOW, "output rows exceed the capacity");
+    over_capacity = write && capacity >= 0 && total > capacity;
   });
+  if (st == QF_OK && over_capacity) {
+    reinterpret_cast<Context*>(c)->last_error = "output rows exceed the capacity";
+    return QF_ERR_OVERFLOW;
+  }
+  return st;
 }
 
 int qf_create_binding_table(qf_context* c, const qf_table* binding, const qf_table* relation,
