@@ -99,11 +99,48 @@ def cpu_baseline(workload, fraction, log_fn):
     npos = min(rows, int(rows * workload["num_positive"] / nb))
     rels = [[c.cpu().numpy() for c in r] for r in workload["relations"]]
     prob = oracle.Problem(binding, npos, rels, workload["literals"])
-    t0 = time.perf_counter()
-    prob.pipeline(radix_bits=5, chunk_size=32768)
-    dt = time.perf_counter() - t0
+    # One pinned host core, as the reference runs single-threaded.
+    saved = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(saved)})
+    try:
+        t0 = time.perf_counter()
+        prob.pipeline(radix_bits=5, chunk_size=32768)
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, saved)
     log_fn("cpu baseline: %d literals over %d binding rows in %.2f s" % (len(workload["literals"]), rows, dt))
     return dt, rows
+
+
+def host_cpu():
+    """The host's CPU model and the number of CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0))}
+
+
+def copy_ceiling(torch, dev, mib=1024, reps=5):
+    """GB/s of a plain device-to-device copy (read + write bytes), best of `reps`."""
+    src = torch.empty(mib * (1 << 20) // 4, dtype=torch.int32, device=dev).fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = 0.0
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2 * src.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del src, dst
+    return best
 
 
 def main():
@@ -115,7 +152,9 @@ def main():
                     default="step")
     ap.add_argument("--dist", choices=["uniform", "zipf"], default="zipf")
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-fraction", type=float, default=1.0 / 256)
+    ap.add_argument("--cpu-fraction", type=float, default=None,
+                    help="binding rows the CPU baseline runs on (default: about 10 s of CPU work: 1/256 of the "
+                         "refinement step, all of the microbench)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
@@ -169,13 +208,21 @@ def main():
                                           use_graph=(mode == "batched" and not args.no_graph), autotune=auto)
         return counts
 
-    # Warm-up (first step partitions and caches the relations: the cold cost).
+    # Cold step: nothing cached, default facts per partition and tile (partitions the relations).
     tc = time.perf_counter()
-    ref_counts = step("batched")
+    ref_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], w["literals"],
+                                          mode="batched", reuse_partitions=True, radix_bits=args.radix_bits,
+                                          tile_rows=args.tile_rows)
     torch.cuda.synchronize()
     cold_s = time.perf_counter() - tc
-    log("cold batched step (with autotuning): %.3f s" % cold_s)
+    log("cold batched step: %.3f s" % cold_s)
+    autotune_s = None
     if not args.no_autotune and not args.radix_bits and not args.tile_rows:
+        ta = time.perf_counter()
+        tuned_counts = step("batched")  # times the autotuner's grid, then runs the choice
+        torch.cuda.synchronize()
+        autotune_s = time.perf_counter() - ta
+        assert np.array_equal(tuned_counts, ref_counts), "autotuned counts differ"
         choice = ctx.autotune_choice()
         pr, tr, best_ms = choice["partition_rows"], choice["tile_rows"], choice["best_ms"]
         tune.update(partition_rows=pr, tile_rows=tr)
@@ -227,6 +274,8 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: %d MiB" % args.copy_calibration)
 
+    copy_gbs = copy_ceiling(torch, dev)
+    log("device copy ceiling: %.0f GB/s" % copy_gbs)
     algo_bytes, num_groups = algorithmic_probe_bytes(w)
     part_bytes = algorithmic_partition_bytes(w)
     part_ms_per_step = phases.get("partition_binding", (0.0, 0))[0] / max(1, args.steps)
@@ -237,10 +286,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        dt, rows = cpu_baseline(w, args.cpu_fraction, log)
+        frac = args.cpu_fraction if args.cpu_fraction is not None else (1.0 / 256 if args.workload == "step" else 1.0)
+        dt, rows = cpu_baseline(w, frac, log)
         nb = int(w["binding"][0].numel())
-        cpu = {"value": L / (dt * nb / rows), "unit": "literals/s", "cores": 1, "kind": "port",
-               "sample": "oracle pipeline (reference algorithm, 1 thread) on the first %d of %d binding rows with "
+        cpu = {"value": L / (dt * nb / rows), "unit": "literals/s", "cores": 1, "kind": "port", **host_cpu(),
+               "sample": "oracle pipeline (reference algorithm, 1 thread pinned to one core) on the first %d of %d binding rows with "
                          "all relations and literals: %.2f s; value scales that time by %d/%d rows"
                          % (rows, nb, dt, nb, rows)}
 
@@ -264,6 +314,8 @@ def main():
         "plain_ms_per_step": (plain_s * 1e3) if plain_s else None,
         "batched_over_plain": (plain_s / batched_s) if plain_s else None,
         "cold_step_ms": cold_s * 1e3,
+        "autotune_ms": (autotune_s * 1e3) if autotune_s is not None else None,
+        "copy_ceiling_GBs": copy_gbs,
         "phase_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in phases.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
