@@ -17,7 +17,7 @@ clauses compared.  `--workload estimator` records join instances from refinement
 the cardinality MLP on the device and reports q-error, presize fallback rate and the time of
 presized joins against the two-pass joins.
 
-Prints one JSON line (see the keys below).  --gpus above 1 is refused: this build runs on one
+Prints one JSON line (see the keys below).  --gpus above 1 is refused: the library runs on one
 GPU.
 """
 import argparse
@@ -164,7 +164,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, help="MiB of a device copy to run once")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU path); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only (the library has no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -35,7 +35,7 @@ void* Pool::get(size_t bytes) {
   } else {
     hipError_t e = hipMalloc(&p, c);
     if (e != hipSuccess) {
-      // Return cached blocks to the driver and retry once.
+      // Return cached blocks to HIP and retry once.
       (void)hipGetLastError();
       for (auto& kv : free_) (void)hipFree(kv.second);
       free_.clear();
