@@ -10,7 +10,8 @@
                   male, female, brother, sister and the uncle/2 target.
 
 Zipf values are ranks k = 1..D drawn with P(k) ~ k^-s, mapped to values through a seeded
-permutation of [0, D).  Generation runs on the GPU when a torch device is given (fast), else on
+permutation of [0, D): one permutation for every column of the microbench (its frequent join
+values coincide on both sides), one per column elsewhere.  Generation runs on the GPU when a torch device is given (fast), else on
 the host with numpy; the same seed gives the same data for a given device kind.
 """
 import itertools
@@ -32,8 +33,11 @@ def _zipf_cdf(num_values, s):
     return c / c[-1]
 
 
-def column_values(n, num_values, s, seed, device=None):
-    """n int32 values over [0, num_values): uniform when s == 0, else Zipf(s) via a permutation."""
+def column_values(n, num_values, s, seed, device=None, perm_seed=None):
+    """n int32 values over [0, num_values): uniform when s == 0, else Zipf(s) ranks mapped to values
+    through a seeded permutation (`perm_seed`, default derived from `seed`; columns that share it
+    share their frequent values)."""
+    ps = (int(seed) * 7919 + 17) if perm_seed is None else int(perm_seed)
     if device is not None:
         import torch
         g = torch.Generator(device=device)
@@ -44,7 +48,7 @@ def column_values(n, num_values, s, seed, device=None):
         u = torch.rand(n, generator=g, device=device, dtype=torch.float64)
         rank = torch.clamp(torch.searchsorted(cdf, u), max=num_values - 1)
         gp = torch.Generator(device="cpu")
-        gp.manual_seed((int(seed) * 7919 + 17) & 0x7FFFFFFFFFFFFFFF)
+        gp.manual_seed(ps & 0x7FFFFFFFFFFFFFFF)
         perm = torch.randperm(num_values, generator=gp).to(device=device, dtype=torch.int32)
         return perm[rank]
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -52,7 +56,7 @@ def column_values(n, num_values, s, seed, device=None):
         return rng.integers(0, num_values, size=n, dtype=np.int64).astype(np.int32)
     cdf = _zipf_cdf(num_values, s)
     rank = np.minimum(np.searchsorted(cdf, rng.random(n)), num_values - 1)
-    perm = np.random.Generator(np.random.PCG64(seed * 7919 + 17)).permutation(num_values).astype(np.int32)
+    perm = np.random.Generator(np.random.PCG64(ps)).permutation(num_values).astype(np.int32)
     return perm[rank]
 
 
@@ -68,8 +72,10 @@ def all_placements(arity, num_vars):
 def microbench(distribution="uniform", rows=MICRO_ROWS, rel_rows=MICRO_REL_ROWS, num_values=MICRO_KEYS,
                device=None, seed=0x51F00000):
     s = 0.0 if distribution == "uniform" else 1.0
-    binding = [column_values(rows, num_values, s, seed + c, device) for c in range(3)]
-    relation = [column_values(rel_rows, num_values, s, seed + 16 + c, device) for c in range(2)]
+    # One permutation for every column on both sides: the frequent join values coincide (the
+    # skewed-join case: about 5.6e11 matching pairs at full size).
+    binding = [column_values(rows, num_values, s, seed + c, device, perm_seed=seed) for c in range(3)]
+    relation = [column_values(rel_rows, num_values, s, seed + 16 + c, device, perm_seed=seed) for c in range(2)]
     num_positive = int(rows * 0.2)
     literals = [(0, [0, -1]), (0, [0, 1]), (0, [0, 2])]
     return {"binding": binding, "num_positive": num_positive, "relations": [relation], "literals": literals,
