@@ -288,18 +288,25 @@ __global__ void k_count_items(const DevBatch* batches, const DevUnit* units, con
   counts[cell] = (rows + tile - 1) / tile;
 }
 
-// Writes the items of every cell at the cell's scanned start; the last slot gets the total.
+// One thread per work item (the grid covers an upper bound of the item count): item i belongs to
+// the last cell whose scanned start is <= i (cells without items share their start with the
+// next cell).  Thread 0 writes the total.  A cell with many items (a frequent join value) is
+// expanded by many threads.
 __global__ void k_expand_items(const int64_t* unit_cells, int num_units, const int64_t* counts,
                                const int64_t* starts, DevItem* items, int64_t* total) {
-  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t cells = unit_cells[num_units];
-  if (cell >= cells) return;
-  const int64_t n = counts[cell];
-  if (cell == cells - 1) *total = starts[cell] + n;
-  if (n == 0) return;
-  const int u = find_unit(unit_cells, num_units, cell);
-  const int64_t q = cell - unit_cells[u];
-  for (int64_t c = 0; c < n; ++c) items[starts[cell] + c] = DevItem{u, (int)c, q};
+  const int64_t all = starts[cells - 1] + counts[cells - 1];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *total = all;
+  if (i >= all) return;
+  int64_t lo = 0, hi = cells - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int u = find_unit(unit_cells, num_units, lo);
+  items[i] = DevItem{u, (int)(i - starts[lo]), lo - unit_cells[u]};
 }
 
 struct EvalArgs {
@@ -1513,9 +1520,9 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
                            P.d_cell_counts.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_cell_counts.as<int64_t>(), P.d_cell_starts.as<int64_t>(), cells);
-        hipLaunchKernelGGL(k_expand_items, grid, dim3(256), 0, ctx->stream, unit_cells, nu,
-                           P.d_cell_counts.as<int64_t>(), P.d_cell_starts.as<int64_t>(), P.d_items.as<DevItem>(),
-                           P.d_num_items.as<int64_t>());
+        hipLaunchKernelGGL(k_expand_items, dim3((unsigned)std::max<int64_t>(1, ceil_div(items_bound, 256))), dim3(256),
+                           0, ctx->stream, unit_cells, nu, P.d_cell_counts.as<int64_t>(),
+                           P.d_cell_starts.as<int64_t>(), P.d_items.as<DevItem>(), P.d_num_items.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
       }
       // Blocks loop over the items the device listed; more blocks than the bound would idle.
