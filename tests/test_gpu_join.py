@@ -1,4 +1,6 @@
 """GPU parity of binding-set extension (row for row) and the coverage semi-joins."""
+import itertools
+
 import numpy as np
 import pytest
 
@@ -117,3 +119,34 @@ def test_example_coverage(qf_ctx, vars_, rows):
     got = [bool((int(b[e >> 6]) >> (e & 63)) & 1) for e in range(200)]
     assert got == want
     assert ones == sum(want)
+
+
+@pytest.mark.parametrize("seed", [41, 42])
+def test_device_resident_binding_lifecycle(qf_ctx, seed):
+    """Two refinement steps with the binding set kept on the device: step 1's counts size the
+    label-aware extension (built on the device), whose columns feed step 2's evaluation directly;
+    both steps' counts and the extension's rows equal the oracle's."""
+    binding, npos, rels, lits = random_problem(seed, n=3000, num_cols=2, rel_shapes=((2, 400), (3, 300)),
+                                               domain=30)
+    dev_rels = [[_t(c) for c in r] for r in rels]
+    counts1, _ = qf_ctx.evaluate_literals([_t(c) for c in binding], npos, dev_rels, lits)
+    prob1 = oracle.Problem(binding, npos, rels, lits)
+    np.testing.assert_array_equal(counts1, prob1.grouped()[0])
+    # The literal with the most binding rows that adds a new variable.
+    cand = [i for i, (r, v) in enumerate(lits) if -1 in v and counts1[i][2] + counts1[i][3] > 0]
+    pick = max(cand, key=lambda i: int(counts1[i][2] + counts1[i][3]))
+    rel, vars_ = lits[pick]
+    ext = qf_ctx.create_label_aware_binding_table([_t(c) for c in binding], npos, dev_rels[rel], vars_,
+                                                  int(counts1[pick][2]), int(counts1[pick][3]))
+    _, want_ext = prob1.label_aware_binding_table(pick, int(counts1[pick][2]), int(counts1[pick][3]))
+    for g, w in zip(ext, want_ext):
+        np.testing.assert_array_equal(g.cpu().numpy(), w)
+    # Step 2 on the extension as it lies in device memory (positives first).
+    npos2 = int(counts1[pick][2])
+    ncols2 = len(ext)
+    lits2 = [(r, v) for r, (arity, _) in enumerate(((2, 400), (3, 300)))
+             for v in itertools.product(range(-1, ncols2), repeat=arity) if any(x >= 0 for x in v)]
+    lits2 = [(r, list(v)) for r, v in lits2]
+    counts2, _ = qf_ctx.evaluate_literals(list(ext), npos2, dev_rels, lits2)
+    prob2 = oracle.Problem([g.cpu().numpy() for g in ext], npos2, rels, lits2)
+    np.testing.assert_array_equal(counts2, prob2.grouped()[0])
