@@ -316,7 +316,7 @@ def main():
         "data": "synthetic (seeded generators)",
         "config": {"workload": workload_name, "state": "warm (relation partitions cached)",
                    "literals": L, "join_groups": num_groups,
-                   "facts_per_partition": tune["partition_rows"] or 512, "tile_rows": tune["tile_rows"] or 1024},
+                   "facts_per_partition": tune["partition_rows"] or 256, "tile_rows": tune["tile_rows"] or 1024},
         "plain_literals_per_s": (L / plain_s) if plain_s else None,
         "plain_ms_per_step": (plain_s * 1e3) if plain_s else None,
         "batched_over_plain": (plain_s / batched_s) if plain_s else None,

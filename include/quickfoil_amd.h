@@ -127,7 +127,7 @@ typedef struct {
                             caches them on the fact table (learner/CandidateLiteralEvaluator.cpp:
                             291-299).  Only for relations that stay unchanged at that address
                             until qf_context_trim; default 0. */
-  int partition_rows;    /* 0 = default (512) or autotuned; target facts per relation partition,
+  int partition_rows;    /* 0 = default (256) or autotuned; target facts per relation partition,
                             from which each relation's radix bits follow */
   int autotune;          /* 1: on the first call for a workload shape, time a grid of
                             (partition_rows, tile_rows) and keep the fastest in the context */

@@ -35,7 +35,7 @@
 namespace qf {
 
 #ifndef QF_EV_THREADS
-#define QF_EV_THREADS 512
+#define QF_EV_THREADS 256
 #endif
 constexpr int EV_THREADS = QF_EV_THREADS;  // threads of one evaluation block (a multiple of 64)
 constexpr int EV_WAVES = EV_THREADS / 64;
@@ -46,7 +46,7 @@ constexpr int EV_MAXTC = 12;           // staged binding columns (plus the row i
 constexpr int EV_MAXG = EV_MAXTC;      // join variables per unit
 constexpr int EV_MAXLU = 512;          // literals per unit
 constexpr int EV_TBL_WORDS = 2048;     // LDS words for one count table (8 KB)
-constexpr int EV_PST_WORDS = 3072;     // LDS words for the partition's facts (12 KB)
+constexpr int EV_PST_WORDS = 1536;     // LDS words for the partition's facts (6 KB)
 constexpr int EV_MIN_CAP = 16;
 constexpr int EV_MAXMASK = 64;         // literals per unit whose hit masks prune other literals
 constexpr int EV_MAXBU = 16;           // units per batch (units of equal radix bits share staged rows)
@@ -118,8 +118,8 @@ __device__ inline int64_t uni64(int64_t v) {
 
 // LDS tables: cap = P2(4 * facts) slots (load factor <= 1/4, so a miss visits ~1.4 slots), or
 // P2(2 * facts) (load <= 1/2) when that would not fit the table region.  One word per slot: fingerprint (10 bits) << 22 | (1 + staged fact row) << 11 | count (11 bits);
-// 0 = empty.  Facts and counts stay below 2048 because an LDS partition holds at most 1024 facts.
-constexpr int EV_LDS_MAX_FACTS = 1024;
+// 0 = empty.  Facts and counts stay below 2048 because an LDS partition holds at most 512 facts.
+constexpr int EV_LDS_MAX_FACTS = 512;
 constexpr uint32_t EV_FP_SHIFT = 22;
 constexpr uint32_t EV_REP_SHIFT = 11;
 constexpr uint32_t EV_FIELD_MASK = 0x7ffu;
@@ -722,8 +722,9 @@ __device__ inline void prefetch_facts(cptr<DevUnit> U, int unit, int64_t q, int6
   pf.q = q;
 }
 
-// Three 512-thread blocks share a CU when their LDS fits (about 50 KB each); that takes at most
-// 80 VGPRs per lane (6 waves per SIMD), which the attribute asks the register allocator for.
+// 256-thread blocks: a block's barriers wait for four waves only, which measured faster than
+// eight-wave blocks at the same LDS per CU.  Up to six such blocks share a CU when their LDS fits;
+// that takes at most 80 VGPRs per lane (6 waves per SIMD), which the attribute asks for.
 #ifndef QF_EV_WAVES_PER_EU
 #define QF_EV_WAVES_PER_EU 6
 #endif
@@ -998,7 +999,7 @@ static void normalise_key(LitSpec& ls) {
 }
 
 static int auto_radix_bits(int64_t rows, int target) {
-  // About `target` facts per partition (default 512): most count tables fit one LDS table.
+  // About `target` facts per partition (default 256): most partitions fit the LDS tables.
   int r = 0;
   while (r < 16 && (rows >> r) > target) ++r;
   return r;
@@ -1193,7 +1194,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   for (const GroupSpec& g : specs) {
     const int R = o.radix_bits > 0 ? std::min(16, o.radix_bits)
                                    : auto_radix_bits(relations[g.relation].num_rows,
-                                                     o.partition_rows > 0 ? o.partition_rows : 512);
+                                                     o.partition_rows > 0 ? o.partition_rows : 256);
     unit_R[std::make_pair(g.relation, g.join_col)] = R;
     P.R_T = std::max(P.R_T, R);
   }
@@ -1597,8 +1598,8 @@ static void run_groups(Context* ctx, const qf_table& binding, int64_t num_positi
   if (it == tc.best.end()) {
     // Grid: facts per relation partition (radix bits) x binding rows per work item.
     double best_ms = 1e30;
-    int best_pr = 512, best_tile = 1024;
-    for (int pr : {256, 512, 1024})
+    int best_pr = 256, best_tile = 1024;
+    for (int pr : {128, 256, 512})
       for (int tile : {512, 1024, 2048}) {
         qf_eval_options t = o;
         t.partition_rows = pr;
