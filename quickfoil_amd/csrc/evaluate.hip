@@ -335,6 +335,7 @@ struct ItemShared {
   int seg_off[EV_MAXG + 1];
   int seg_rows[EV_MAXG];
   int64_t seg_src[EV_MAXG];
+  const int32_t* seg_tid[EV_MAXG];  // the segment's first binding row id (HBM), for covered bitvectors
   int seg_hits[EV_MAXG];       // rows of the segment whose join value occurs in the partition
   int blk_start[EV_MAXG + 1];  // join phase: 64-row blocks of all rows, per segment
   int ent_start[EV_MAXG + 1];  // subset phase: (literal, block of hit rows) entries, per segment
@@ -456,7 +457,8 @@ __device__ inline void lds_probe1_x2(const uint32_t* tbl, uint32_t cap, const in
 // literal.  The literals keyed on the join column alone (subset 0) are counted here.  A wave
 // takes two 64-row blocks of a segment per step.
 template <bool kLds>
-__device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, const int32_t* tstage, uint16_t* cidx,
+__device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, const int32_t* tstage,
+                           const uint8_t* posf, uint16_t* cidx,
                            const uint32_t* tbl, const uint32_t* cnt, const int32_t* keys, uint32_t cap,
                            const int32_t* pstage, int np, int lane, int wave) {
   const int nseg = uni(sh.nseg);
@@ -464,7 +466,6 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
   const int e0 = (int)((int64_t)E * wave / EV_WAVES), e1 = (int)((int64_t)E * (wave + 1) / EV_WAVES);
   if (e0 >= e1) return;
   const int tile = a.tile_rows;
-  const int32_t* tid_col = tstage + a.ntc * tile;
   cptr<DevLit> lits = as_const(a.lits);
   const int32_t* pkey = pstage + U->jcol * np;
   int sg = 0;
@@ -496,18 +497,20 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
       if (ca != 0) cidx[seg_off + base + (int)lane_prefix(ma, lane)] = (uint16_t)ra;
       if (cb != 0) cidx[seg_off + base + __popcll(ma) + (int)lane_prefix(mb, lane)] = (uint16_t)rb;
       if (l1 > l0) {
-        const int32_t ta = tid_col[row_a], tb = tid_col[row_b];
-        const bool pa = ta < a.num_positive, pb = tb < a.num_positive;
+        const bool pa = posf[row_a] != 0, pb = posf[row_b] != 0;
         cp += __popcll(__ballot(ca != 0 && pa)) + __popcll(__ballot(cb != 0 && pb));
         cn += __popcll(__ballot(ca != 0 && !pa)) + __popcll(__ballot(cb != 0 && !pb));
         lbp += (pa ? ca : 0u) + (pb ? cb : 0u);
         lbn += (pa ? 0u : ca) + (pb ? 0u : cb);
-        if (a.bits != nullptr)
+        if (a.bits != nullptr) {
+          const int32_t* tids = sh.seg_tid[sg];
+          const int32_t ta = ca != 0 ? tids[row_a - seg_off] : 0, tb = cb != 0 ? tids[row_b - seg_off] : 0;
           for (int li = l0; li < l1; ++li) {
             unsigned long long* bits = a.bits + (int64_t)lits[li].slot * a.bit_words;
             if (ca != 0) atomicOr(&bits[ta >> 6], 1ull << (ta & 63));
             if (cb != 0) atomicOr(&bits[tb >> 6], 1ull << (tb & 63));
           }
+        }
       }
     }
     const int step = two ? 2 : 1;
@@ -539,7 +542,7 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
 // else the HBM table (cnt, keys).  NK > 0 fixes the key width; NK == 0 reads it from `nk`.
 template <int NK, bool kLds>
 __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk,
-                              const int32_t* tstage, const uint16_t* cidx, uint64_t* masks, const uint32_t* tbl,
+                              const int32_t* tstage, const uint8_t* posf, const uint16_t* cidx, uint64_t* masks, const uint32_t* tbl,
                               const uint32_t* cnt, const int32_t* keys, uint32_t cap, const int32_t* pstage, int np,
                               int lane, int wave) {
   const int nseg = uni(sh.nseg);
@@ -548,7 +551,6 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
   if (e0 >= e1) return;
   const int tile = a.tile_rows;
   const int mstride = a.mstride;
-  const int32_t* tid_col = tstage + a.ntc * tile;
   cptr<DevLit> lits = as_const(a.lits);
   int poff[EV_MAXK];
 #pragma unroll
@@ -592,14 +594,15 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
         }
       }
       if (__ballot(c != 0)) {
-        const int32_t tid = tid_col[row];
-        const bool pos = tid < a.num_positive;
+        const bool pos = posf[row] != 0;
         cp += __popcll(__ballot(c != 0 && pos));
         cn += __popcll(__ballot(c != 0 && !pos));
         lbp += pos ? c : 0u;
         lbn += pos ? 0u : c;
-        if (a.bits != nullptr && c != 0)
+        if (a.bits != nullptr && c != 0) {
+          const int32_t tid = sh.seg_tid[sg][row - seg_off];
           atomicOr(&a.bits[(int64_t)L.slot * a.bit_words + (tid >> 6)], 1ull << (tid & 63));
+        }
       }
     }
     if (L.mslot >= 0) {
@@ -660,7 +663,8 @@ __device__ void build_lds_table(uint32_t* tbl, uint32_t cap, const int32_t* psta
 // cleared before the barrier that precedes this call), then probe.
 template <int NK>
 __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk, int hq,
-                                   uint32_t* tbl, const int32_t* tstage, const uint16_t* cidx, uint64_t* masks,
+                                   uint32_t* tbl, const int32_t* tstage, const uint8_t* posf, const uint16_t* cidx,
+                                   uint64_t* masks,
                                    const int32_t* pstage, int np, int lane, int wave) {
   if (hq < 0) {
     const uint32_t cap = table_cap_lds(np);
@@ -670,11 +674,12 @@ __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const Ite
       if (NK || k < nk) poff[k] = U->sub[s].pcol[k] * np;
     build_lds_table<NK>(tbl, cap, pstage, poff, np, nk);
     __syncthreads();  // table ready
-    probe_entries<NK, true>(a, U, sh, s, nk, tstage, cidx, masks, tbl, nullptr, nullptr, cap, pstage, np, lane, wave);
+    probe_entries<NK, true>(a, U, sh, s, nk, tstage, posf, cidx, masks, tbl, nullptr, nullptr, cap, pstage, np, lane,
+                            wave);
   } else {
     const uint32_t cap = table_cap_hbm(np);
     const uint32_t* base = U->sub[s].hbuf + U->sub[s].hbase[hq];
-    probe_entries<NK, false>(a, U, sh, s, nk, tstage, cidx, masks, nullptr, base + cap,
+    probe_entries<NK, false>(a, U, sh, s, nk, tstage, posf, cidx, masks, nullptr, base + cap,
                              reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap), cap, pstage, np, lane, wave);
   }
 }
@@ -733,7 +738,8 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
   extern __shared__ uint32_t lds[];
   uint32_t* tbl = lds;                                                                 // EV_TBL_WORDS
   int32_t* pstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS);                   // EV_PST_WORDS
-  int32_t* tstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS + EV_PST_WORDS);   // (ntc + 1) x tile
+  int32_t* tstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS + EV_PST_WORDS);   // ntc x tile
+  uint8_t* posf = reinterpret_cast<uint8_t*>(tstage + a.ntc * a.tile_rows);          // tile bytes
   uint16_t* cidx = reinterpret_cast<uint16_t*>(lds + a.cidx_off);                    // tile
   uint64_t* masks = reinterpret_cast<uint64_t*>(lds + a.mask_off);                   // slots x mstride
   __shared__ ItemShared sh;
@@ -777,6 +783,7 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
         sh.seg_off[idx] = off;
         sh.seg_rows[idx] = n;
         sh.seg_src[idx] = t0 + (lo_ - run);
+        sh.seg_tid[idx] = B->tcols[g][ntc] + (t0 + (lo_ - run));
       }
       if (lane == 63) {
         sh.seg_off[__popcll(has)] = off + n;
@@ -803,10 +810,15 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
       const int g = uni(sh.seg_group[sgi]);
       const int64_t src0 = uni64(sh.seg_src[sgi]);
       const int n = uni(sh.seg_rows[sgi]), dst0 = uni(sh.seg_off[sgi]);
-      for (int c = 0; c <= ntc; ++c) {
+      for (int c = 0; c < ntc; ++c) {
         const int32_t* src = B->tcols[g][c] + src0;
         int32_t* dst = tstage + c * a.tile_rows + dst0;
         for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
+      }
+      {  // row ids become positive flags (a row is positive below num_positive)
+        const int32_t* src = B->tcols[g][ntc] + src0;
+        uint8_t* dst = posf + dst0;
+        for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i] < a.num_positive ? 1 : 0;
       }
     }
 
@@ -874,11 +886,11 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
       const int poff = U->jcol * np;
       build_lds_table<1>(tbl, cap, pstage, &poff, np, 1);
       __syncthreads();  // table ready
-      join_phase<true>(a, U, sh, tstage, cidx, tbl, nullptr, nullptr, cap, pstage, np, lane, wave);
+      join_phase<true>(a, U, sh, tstage, posf, cidx, tbl, nullptr, nullptr, cap, pstage, np, lane, wave);
     } else {
       const uint32_t cap = table_cap_hbm(np);
       const uint32_t* base = U->sub[0].hbuf + U->sub[0].hbase[hq];
-      join_phase<false>(a, U, sh, tstage, cidx, nullptr, base + cap,
+      join_phase<false>(a, U, sh, tstage, posf, cidx, nullptr, base + cap,
                         reinterpret_cast<const int32_t*>(base + 2 * (int64_t)cap), cap, pstage, np, lane, wave);
     }
 
@@ -912,10 +924,10 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
       if (uni(sh.ent_start[nseg]) == 0 || uni(sh.need[s]) == 0) continue;
       const int nk = U->sub[s].nk;
       switch (nk) {
-        case 1: eval_subset<1>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
-        case 2: eval_subset<2>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
-        case 3: eval_subset<3>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
-        default: eval_subset<0>(a, U, sh, s, nk, hq, tbl, tstage, cidx, masks, pstage, np, lane, wave); break;
+        case 1: eval_subset<1>(a, U, sh, s, nk, hq, tbl, tstage, posf, cidx, masks, pstage, np, lane, wave); break;
+        case 2: eval_subset<2>(a, U, sh, s, nk, hq, tbl, tstage, posf, cidx, masks, pstage, np, lane, wave); break;
+        case 3: eval_subset<3>(a, U, sh, s, nk, hq, tbl, tstage, posf, cidx, masks, pstage, np, lane, wave); break;
+        default: eval_subset<0>(a, U, sh, s, nk, hq, tbl, tstage, posf, cidx, masks, pstage, np, lane, wave); break;
       }
     }
    }
@@ -1459,9 +1471,9 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.d_items = DevBuf(ctx, std::max<int64_t>(1, P.max_items) * sizeof(DevItem));
   P.d_num_items = DevBuf(ctx, 8);
   P.d_counts = DevBuf(ctx, std::max(1, num_slots) * 4 * sizeof(uint64_t));
-  // LDS words: table | facts | staged rows ((ntc + 1) x tile) | hit-row list (uint16 x tile) |
-  // hit masks (uint64 x max_mask x tile / 64).
-  P.cidx_off = EV_TBL_WORDS + EV_PST_WORDS + (int)(P.bcols.size() + 1) * P.tile;
+  // LDS words: table | facts | staged rows (ntc x tile) | positive flags (uint8 x tile) |
+  // hit-row list (uint16 x tile) | hit masks (uint64 x max_mask x tile / 64).
+  P.cidx_off = EV_TBL_WORDS + EV_PST_WORDS + (int)P.bcols.size() * P.tile + P.tile / 4;
   P.mask_off = P.cidx_off + P.tile / 2;
   P.mstride = P.tile / 64;
   P.lds_bytes = ((size_t)P.mask_off + (size_t)P.max_mask * P.mstride * 2) * 4;
@@ -1600,7 +1612,7 @@ static void run_groups(Context* ctx, const qf_table& binding, int64_t num_positi
     double best_ms = 1e30;
     int best_pr = 256, best_tile = 1024;
     for (int pr : {128, 256, 512})
-      for (int tile : {512, 1024, 2048}) {
+      for (int tile : {512, 768, 1024, 1536, 2048}) {
         qf_eval_options t = o;
         t.partition_rows = pr;
         t.tile_rows = tile;
