@@ -48,6 +48,22 @@ class QfLiteral(ctypes.Structure):
     _fields_ = [("relation", ctypes.c_int), ("vars", ctypes.c_int * MAX_ARITY)]
 
 
+class LiteralArray:
+    """Literals converted once to the C layout (qf_literal: relation, then MAX_ARITY variables),
+    for callers that evaluate the same literal list repeatedly."""
+
+    def __init__(self, literals):
+        rows = [[rel] + list(vars_) + [-1] * (MAX_ARITY - len(vars_)) for rel, vars_ in literals]
+        self.array = np.ascontiguousarray(np.array(rows, dtype=np.int32).reshape(-1, 1 + MAX_ARITY))
+        self.count = len(literals)
+
+    def __len__(self):
+        return self.count
+
+    def pointer(self):
+        return self.array.ctypes.data_as(ctypes.POINTER(QfLiteral))
+
+
 class QfEvalOptions(ctypes.Structure):
     _fields_ = [
         ("mode", ctypes.c_int),
@@ -278,20 +294,17 @@ class Context:
                           semi_bits=False, check_range=False, partition_rows=0, autotune=False):
         """Returns (counts[L,4] uint64 ndarray, bits tensor or None).
 
-        literals: list of (relation index, [var per position]) with -1 for new variables.
-        Counts are {covered_pos, covered_neg, binding_pos, binding_neg}.
+        literals: list of (relation index, [var per position]) with -1 for new variables, or a
+        LiteralArray made from one.  Counts are {covered_pos, covered_neg, binding_pos, binding_neg}.
         """
         t = self.torch
         b = _TableHolder(binding_columns)
         rels = [_TableHolder(r) for r in relations]
         rel_arr = (QfTable * max(1, len(rels)))(*[r.t for r in rels])
+        if not isinstance(literals, LiteralArray):
+            literals = LiteralArray(literals)
         L = len(literals)
-        lit_arr = (QfLiteral * max(1, L))()
-        for i, (rel, vars_) in enumerate(literals):
-            lit_arr[i].relation = rel
-            vv = list(vars_) + [-1] * (MAX_ARITY - len(vars_))
-            for k in range(MAX_ARITY):
-                lit_arr[i].vars[k] = vv[k]
+        lit_arr = literals.pointer() if L > 0 else (QfLiteral * 1)()
         types = None
         types_ptr = ctypes.c_void_p(0)
         if arg_types is not None:
