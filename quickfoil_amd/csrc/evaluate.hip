@@ -1557,7 +1557,7 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
 }
 
 struct GraphCache {
-  std::string key;
+  std::vector<int64_t> key;
   std::unique_ptr<StepPlan> plan;
 };
 
@@ -1639,23 +1639,24 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
   if (num_slots == 0) return;
   // A captured step is replayed when the same call repeats: same binding and relation
   // columns, literal set, options and output pointer.
-  std::string key;
+  // The key lists every input of the step (sizes, options, column addresses, literal keys),
+  // with a length before each variable-length part so that different steps never share a key.
+  std::vector<int64_t> key;
   QF_REQUIRE(!o.use_graph || o.reuse_partitions, "use_graph needs reuse_partitions (relations must stay unchanged)");
   if (o.use_graph && o.mode == QF_EVAL_BATCHED) {
-    char buf[128];
-    snprintf(buf, sizeof(buf), "%lld/%lld/%d/%d/%d/%d/%p|", (long long)binding.num_rows, (long long)num_positive,
-             o.radix_bits, o.tile_rows, o.partition_rows, o.reuse_partitions, (void*)d_bits);
-    key = buf;
-    for (int c = 0; c < binding.num_columns; ++c) key += std::to_string((uintptr_t)binding.columns[c]) + ",";
+    key.reserve(64 + 16 * (size_t)num_slots);
+    key.insert(key.end(), {binding.num_rows, num_positive, o.radix_bits, o.tile_rows, o.partition_rows,
+                           o.reuse_partitions, (int64_t)(uintptr_t)d_bits, binding.num_columns});
+    for (int c = 0; c < binding.num_columns; ++c) key.push_back((int64_t)(uintptr_t)binding.columns[c]);
+    key.push_back((int64_t)specs.size());
     for (const GroupSpec& g : specs) {
-      key += std::to_string((uintptr_t)relations[g.relation].columns[0]) + ":" + std::to_string(g.join_col) + ":" +
-             std::to_string(g.join_var) + "[";
+      key.insert(key.end(), {(int64_t)(uintptr_t)relations[g.relation].columns[0], g.join_col, g.join_var,
+                             (int64_t)g.lits.size()});
       for (const LitSpec& l : g.lits) {
-        key += std::to_string(l.slot) + "/";
-        for (size_t k = 0; k < l.pcols.size(); ++k) key += std::to_string(l.pcols[k]) + "=" + std::to_string(l.bcols[k]) + ";";
-        for (auto& e : l.eqs) key += "e" + std::to_string(e.first) + "," + std::to_string(e.second) + ";";
+        key.insert(key.end(), {l.slot, (int64_t)l.pcols.size(), (int64_t)l.eqs.size()});
+        for (size_t k = 0; k < l.pcols.size(); ++k) key.insert(key.end(), {l.pcols[k], l.bcols[k]});
+        for (auto& e : l.eqs) key.insert(key.end(), {e.first, e.second});
       }
-      key += "]";
     }
     if (!ctx->graph_stream) {
       QF_CHECK_HIP(hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking));
