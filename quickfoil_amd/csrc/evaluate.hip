@@ -16,15 +16,21 @@
 // covers the contiguous binding partitions [q << (R_T - R_u), (q + 1) << (R_T - R_u)).
 //
 // Work.  A unit is (relation, join column) with the literal groups of every join variable that
-// uses it (one literal for the plain form).  A work item is (unit, relation partition q, chunk of
-// <= tile binding rows of q across the unit's join variables).  Item counts per (unit, q) are
-// computed and scanned on the device and items are found by binary search.  An item stages its
-// binding rows in LDS; for each key subset it builds an open-addressing count table of the
-// partition's facts in LDS (or uses the HBM table of a heavy partition, built once per
-// (relation, join column, subset) and cached), then the four waves split the (literal, 64-row
-// block) entries of that subset in literal-major order.  Per block: ballots give the covered
-// counts and, when no lane has cnt > 1, the binding counts; otherwise a shuffle reduction sums
-// cnt.  Counts live in wave-uniform registers and are flushed with one atomic per literal run.
+// uses it (one literal for the plain form).  Units with the same radix bits form a batch: they
+// see the same binding cells, so a work item (batch, cell q, chunk of <= tile binding rows of q
+// across the batch's join variables) stages its binding rows in LDS once for all its units.
+// Item counts per cell are computed and scanned on the device and expanded into an item list
+// that a persistent grid of 256-thread workgroups walks.  Per unit, an item first probes a table
+// of the partition's join column for every staged row and lists the rows that match (every
+// literal of the unit binds the join column, so the other rows match nothing), counting the
+// literals keyed on the join column alone; then, per wider key subset, it builds an
+// open-addressing count table of the partition's facts in LDS (or uses the HBM table of a heavy
+// partition, built once per (relation, join column, subset) and cached) and the four waves split
+// the (literal, 64-row block of listed rows) entries of that subset literal-major.  A literal of
+// three or more key columns skips the blocks where its parent (its key minus one column) matched
+// nothing.  Covered counts come from ballots; binding counts from per-lane sums reduced once per
+// literal run and added by four lanes into the workgroup's copy of the counters, which are
+// summed after the launches.
 #include "qf_internal.h"
 
 #include <algorithm>
