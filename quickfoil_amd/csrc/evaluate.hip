@@ -130,6 +130,23 @@ constexpr uint32_t EV_FP_SHIFT = 22;
 constexpr uint32_t EV_REP_SHIFT = 11;
 constexpr uint32_t EV_FIELD_MASK = 0x7ffu;
 static_assert(EV_LDS_MAX_FACTS <= (int)EV_FIELD_MASK, "fact rows and counts must fit 11 bits");
+constexpr uint32_t EV_FP_MASK = ~((1u << EV_FP_SHIFT) - 1u);
+
+// Hash of the LDS tables: one 32-bit multiply per key column.  Only the table's own build and
+// probes see it, and every fingerprint hit is checked against the staged key, so it needs spread
+// rather than the HBM tables' full mixing.  The fingerprint is the product's top 10 bits, already
+// in their place in a slot word; the slot is the log2(cap) bits below them (cap <= 2048: bits
+// 11-21 at most), which every key bit below them feeds.
+__device__ inline uint32_t lds_hash(const int32_t* v, int nk) {
+  uint32_t h = (uint32_t)v[0] * 0x9e3779b1u;
+  for (int k = 1; k < nk; ++k) h = (h ^ (uint32_t)v[k]) * 0x85ebca6bu;
+  return h;
+}
+__device__ inline uint32_t lds_slot(uint32_t h, uint32_t cap) {
+  return (h >> (EV_FP_SHIFT - __builtin_ctz(cap))) & (cap - 1);
+}
+__device__ inline uint32_t lds_fp(uint32_t h) { return h & EV_FP_MASK; }
+static_assert(EV_TBL_WORDS <= (1 << EV_FP_SHIFT), "an LDS table's slot bits sit below the fingerprint");
 __host__ __device__ inline uint32_t table_cap_lds(int64_t np) {
   uint32_t c = next_pow2_u32((uint32_t)(4 * np));
   if (c > (uint32_t)EV_TBL_WORDS) c = next_pow2_u32((uint32_t)(2 * np));
@@ -359,13 +376,13 @@ struct ItemShared {
 template <int NK>
 __device__ inline uint32_t lds_probe(const uint32_t* tbl, uint32_t cap, const int32_t* pstage, const int* poff,
                                      const int32_t* vals, int nk) {
-  const uint32_t h = tuple_hash(vals, NK ? NK : nk);
-  const uint32_t fp = h >> EV_FP_SHIFT;
-  uint32_t slot = h & (cap - 1);
+  const uint32_t h = lds_hash(vals, NK ? NK : nk);
+  const uint32_t fp = lds_fp(h);
+  uint32_t slot = lds_slot(h, cap);
   while (true) {
     const uint32_t w = tbl[slot];
     if (w == 0) return 0;
-    if ((w >> EV_FP_SHIFT) == fp) {
+    if ((w & EV_FP_MASK) == fp) {
       const int r = (int)((w >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1;
       bool eq = true;
 #pragma unroll
@@ -438,16 +455,16 @@ __device__ inline uint32_t lane_prefix(uint64_t m, int lane) {
 // flight together.  Finished lanes keep reading in-range words.
 __device__ inline void lds_probe1_x2(const uint32_t* tbl, uint32_t cap, const int32_t* pkey, int32_t va, int32_t vb,
                                      bool live_a, bool live_b, uint32_t& ca, uint32_t& cb) {
-  const uint32_t ha = tuple_hash(&va, 1), hb = tuple_hash(&vb, 1);
-  const uint32_t fa = ha >> EV_FP_SHIFT, fb = hb >> EV_FP_SHIFT;
-  uint32_t sa = ha & (cap - 1), sb = hb & (cap - 1);
+  const uint32_t ha = lds_hash(&va, 1), hb = lds_hash(&vb, 1);
+  const uint32_t fa = lds_fp(ha), fb = lds_fp(hb);
+  uint32_t sa = lds_slot(ha, cap), sb = lds_slot(hb, cap);
   ca = cb = 0;
   while (live_a || live_b) {
     const uint32_t wa = tbl[sa], wb = tbl[sb];
     const int ra = (int)((wa >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1, rb = (int)((wb >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1;
     const int32_t ka = pkey[ra < 0 ? 0 : ra], kb = pkey[rb < 0 ? 0 : rb];
-    const bool hit_a = wa != 0 && (wa >> EV_FP_SHIFT) == fa && ka == va;
-    const bool hit_b = wb != 0 && (wb >> EV_FP_SHIFT) == fb && kb == vb;
+    const bool hit_a = wa != 0 && (wa & EV_FP_MASK) == fa && ka == va;
+    const bool hit_b = wb != 0 && (wb & EV_FP_MASK) == fb && kb == vb;
     if (live_a && hit_a) ca = wa & EV_FIELD_MASK;
     if (live_b && hit_b) cb = wb & EV_FIELD_MASK;
     live_a = live_a && wa != 0 && !hit_a;
@@ -644,14 +661,14 @@ __device__ void build_lds_table(uint32_t* tbl, uint32_t cap, const int32_t* psta
 #pragma unroll
     for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
       if (NK || k < nk) vals[k] = pstage[poff[k] + r];
-    const uint32_t h = tuple_hash(vals, NK ? NK : nk);
-    const uint32_t fp = h >> EV_FP_SHIFT;
-    const uint32_t mine = (fp << EV_FP_SHIFT) | ((uint32_t)(r + 1) << EV_REP_SHIFT);
-    uint32_t slot = h & (cap - 1);
+    const uint32_t h = lds_hash(vals, NK ? NK : nk);
+    const uint32_t fp = lds_fp(h);
+    const uint32_t mine = fp | ((uint32_t)(r + 1) << EV_REP_SHIFT);
+    uint32_t slot = lds_slot(h, cap);
     while (true) {
       const uint32_t prev = atomicCAS(&tbl[slot], 0u, mine);
       if (prev == 0) break;
-      if ((prev >> EV_FP_SHIFT) == fp) {
+      if ((prev & EV_FP_MASK) == fp) {
         const int rep = (int)((prev >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1;
         bool eq = true;
 #pragma unroll
