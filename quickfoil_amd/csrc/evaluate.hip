@@ -394,13 +394,13 @@ __device__ inline uint32_t lds_probe(const uint32_t* tbl, uint32_t cap, const in
   }
 }
 
-// Wave-uniform registers of the literal being probed.
+// Wave-uniform registers of the literal being probed.  Its row-side equalities (rare: only a
+// plan that binds one relation column to two binding columns has them) are read when needed.
 struct LitRegs {
   int slot;
   int neq;
   int pmask, mslot;
   int tb[EV_MAXK];  // word offsets of the key's binding columns in the staged tile
-  int eqa[EV_MAXEQ], eqb[EV_MAXEQ];
 };
 
 template <int NK>
@@ -413,12 +413,14 @@ __device__ inline void load_literal(cptr<DevLit> lits, int li, int nk, int tile,
 #pragma unroll
   for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
     if (NK || k < nk) L.tb[k] = p->tcol[k] * tile;
-#pragma unroll
-  for (int q = 0; q < EV_MAXEQ; ++q)
-    if (q < L.neq) {
-      L.eqa[q] = p->eq[q][0] * tile;
-      L.eqb[q] = p->eq[q][1] * tile;
-    }
+}
+
+// Whether staged row `row` meets literal li's row-side equalities.
+__device__ inline bool row_equalities(cptr<DevLit> lits, int li, int neq, int tile, const int32_t* tstage, int row) {
+  cptr<DevLit> p = lits + li;
+  bool ok = true;
+  for (int q = 0; q < neq; ++q) ok &= tstage[p->eq[q][0] * tile + row] == tstage[p->eq[q][1] * tile + row];
+  return ok;
 }
 
 // Per-lane binding sums: 32 bits for LDS tables (a count is below 2^11 and a run covers at most
@@ -601,12 +603,7 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
       const bool valid = rloc < seg_hits && ((pm >> lane) & 1);
       const int row = seg_off + (rloc < seg_hits ? (int)cidx[seg_off + rloc] : 0);
       if (valid) {
-        bool ok = true;
-        if (L.neq > 0) {
-#pragma unroll
-          for (int q = 0; q < EV_MAXEQ; ++q)
-            if (q < L.neq) ok &= tstage[L.eqa[q] + row] == tstage[L.eqb[q] + row];
-        }
+        const bool ok = L.neq == 0 || row_equalities(lits, lit0 + lit, L.neq, tile, tstage, row);
         if (ok) {
           int32_t vals[EV_MAXK];
 #pragma unroll
@@ -756,8 +753,12 @@ __device__ inline void prefetch_facts(cptr<DevUnit> U, int unit, int64_t q, int6
 #ifndef QF_EV_WAVES_PER_EU
 #define QF_EV_WAVES_PER_EU 6
 #endif
+// kBits: covered bitvectors were asked for (a.bits); the instance without them compiles their
+// code out of the probe loops.
+template <bool kBits>
 __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_EV_WAVES_PER_EU))) void k_eval(
     EvalArgs a) {
+  if (!kBits) a.bits = nullptr;
   extern __shared__ uint32_t lds[];
   uint32_t* tbl = lds;                                                                 // EV_TBL_WORDS
   int32_t* pstage = reinterpret_cast<int32_t*>(lds + EV_TBL_WORDS);                   // EV_PST_WORDS
@@ -1501,10 +1502,12 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.mstride = P.tile / 64;
   P.lds_bytes = ((size_t)P.mask_off + (size_t)P.max_mask * P.mstride * 2) * 4;
   QF_REQUIRE(P.lds_bytes <= 160 * 1024 - 4096, "work item does not fit in LDS");
-  QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_eval), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)P.lds_bytes));
-  int per_cu = 0;
-  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eval, EV_THREADS, P.lds_bytes));
+  for (const void* fn : {reinterpret_cast<const void*>(k_eval<false>), reinterpret_cast<const void*>(k_eval<true>)})
+    QF_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds_bytes));
+  int per_cu = 0, per_cu_bits = 0;
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eval<false>, EV_THREADS, P.lds_bytes));
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_bits, k_eval<true>, EV_THREADS, P.lds_bytes));
+  per_cu = std::min(per_cu, per_cu_bits);
   P.max_blocks = ctx->num_cus * std::max(1, per_cu);
   // Counter copies: up to one per CU, within 32 MiB.
   const int64_t copy_bytes = (int64_t)std::max(1, num_slots) * 4 * (int64_t)sizeof(uint64_t);
@@ -1564,7 +1567,10 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
       // Blocks loop over the items the device listed; more blocks than the bound would idle.
       const int blocks = (int)std::min<int64_t>(P.max_blocks, std::max<int64_t>(1, items_bound));
       PhaseScope ps(ctx, "probe_count");
-      hipLaunchKernelGGL(k_eval, dim3(blocks), dim3(EV_THREADS), P.lds_bytes, ctx->stream, a);
+      if (a.bits != nullptr)
+        hipLaunchKernelGGL(k_eval<true>, dim3(blocks), dim3(EV_THREADS), P.lds_bytes, ctx->stream, a);
+      else
+        hipLaunchKernelGGL(k_eval<false>, dim3(blocks), dim3(EV_THREADS), P.lds_bytes, ctx->stream, a);
       QF_CHECK_HIP(hipGetLastError());
     }
   }
