@@ -122,9 +122,11 @@ __device__ inline int64_t uni64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// LDS tables: cap = P2(4 * facts) slots (load factor <= 1/4, so a miss visits ~1.4 slots), or
-// P2(2 * facts) (load <= 1/2) when that would not fit the table region.  One word per slot: fingerprint (10 bits) << 22 | (1 + staged fact row) << 11 | count (11 bits);
-// 0 = empty.  Facts and counts stay below 2048 because an LDS partition holds at most 512 facts.
+// LDS tables: cap = P2(8 * facts) slots (load factor <= 1/8) when that fits the table region, else
+// P2(4 * facts) (load <= 1/4), else P2(2 * facts).  A wave's probe loop runs as long as its
+// longest probe: over ~50 lanes that is ~3 slots at load 1/8 and ~4.6 at load 1/4 (a miss
+// averages 1.15 and 1.4 slots).  One word per slot: fingerprint (10 bits) << 22 | (1 + staged
+// fact row) << 11 | count (11 bits); 0 = empty.  Facts and counts stay below 2048 because an LDS partition holds at most 512 facts.
 constexpr int EV_LDS_MAX_FACTS = 512;
 constexpr uint32_t EV_FP_SHIFT = 22;
 constexpr uint32_t EV_REP_SHIFT = 11;
@@ -148,7 +150,8 @@ __device__ inline uint32_t lds_slot(uint32_t h, uint32_t cap) {
 __device__ inline uint32_t lds_fp(uint32_t h) { return h & EV_FP_MASK; }
 static_assert(EV_TBL_WORDS <= (1 << EV_FP_SHIFT), "an LDS table's slot bits sit below the fingerprint");
 __host__ __device__ inline uint32_t table_cap_lds(int64_t np) {
-  uint32_t c = next_pow2_u32((uint32_t)(4 * np));
+  uint32_t c = next_pow2_u32((uint32_t)(8 * np));
+  if (c > (uint32_t)EV_TBL_WORDS) c = next_pow2_u32((uint32_t)(4 * np));
   if (c > (uint32_t)EV_TBL_WORDS) c = next_pow2_u32((uint32_t)(2 * np));
   return c < EV_MIN_CAP ? EV_MIN_CAP : c;
 }
