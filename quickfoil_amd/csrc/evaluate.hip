@@ -122,11 +122,12 @@ __device__ inline int64_t uni64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// LDS tables: cap = P2(8 * facts) slots (load factor <= 1/8) when that fits the table region, else
-// P2(4 * facts) (load <= 1/4), else P2(2 * facts).  A wave's probe loop runs as long as its
-// longest probe: over ~50 lanes that is ~3 slots at load 1/8 and ~4.6 at load 1/4 (a miss
+// LDS tables: every table takes the whole table region, EV_LDS_CAP = 2048 slots, so its load
+// factor is at most 1/4 (512 facts) and 1/8 at the usual 256.  A wave's probe loop runs as long
+// as its longest probe: over ~50 lanes that is ~3 slots at load 1/8 and ~4.6 at load 1/4 (a miss
 // averages 1.15 and 1.4 slots).  One word per slot: fingerprint (10 bits) << 22 | (1 + staged
-// fact row) << 11 | count (11 bits); 0 = empty.  Facts and counts stay below 2048 because an LDS partition holds at most 512 facts.
+// fact row) << 11 | count (11 bits); 0 = empty.  Facts and counts stay below 2048 because an LDS
+// partition holds at most 512 facts.
 constexpr int EV_LDS_MAX_FACTS = 512;
 constexpr uint32_t EV_FP_SHIFT = 22;
 constexpr uint32_t EV_REP_SHIFT = 11;
@@ -145,15 +146,19 @@ __device__ inline uint32_t lds_hash(const int32_t* v, int nk) {
   return h;
 }
 __device__ inline uint32_t lds_slot(uint32_t h, uint32_t cap) {
-  return (h >> (EV_FP_SHIFT - __builtin_ctz(cap))) & (cap - 1);
+  return (h >> (EV_FP_SHIFT - __builtin_ctz(cap))) & (cap - 1);  // cap is a power of two
 }
 __device__ inline uint32_t lds_fp(uint32_t h) { return h & EV_FP_MASK; }
 static_assert(EV_TBL_WORDS <= (1 << EV_FP_SHIFT), "an LDS table's slot bits sit below the fingerprint");
-__host__ __device__ inline uint32_t table_cap_lds(int64_t np) {
-  uint32_t c = next_pow2_u32((uint32_t)(8 * np));
-  if (c > (uint32_t)EV_TBL_WORDS) c = next_pow2_u32((uint32_t)(4 * np));
-  if (c > (uint32_t)EV_TBL_WORDS) c = next_pow2_u32((uint32_t)(2 * np));
-  return c < EV_MIN_CAP ? EV_MIN_CAP : c;
+constexpr uint32_t EV_LDS_CAP = EV_TBL_WORDS;
+static_assert((EV_LDS_CAP & (EV_LDS_CAP - 1)) == 0 && EV_LDS_CAP >= 4 * EV_LDS_MAX_FACTS,
+              "LDS tables are a power of two, at load <= 1/4");
+static_assert(EV_LDS_CAP % (4 * QF_EV_THREADS) == 0, "a block clears a table in 16-byte words");
+// Clears an LDS table (16 bytes per thread and step; the table starts the block's LDS).
+__device__ inline void clear_lds_table(uint32_t* tbl) {
+#pragma unroll
+  for (int i = threadIdx.x; i < (int)EV_LDS_CAP / 4; i += QF_EV_THREADS)
+    reinterpret_cast<uint4*>(tbl)[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 __host__ __device__ inline uint32_t table_cap_hbm(int64_t np) {
   uint32_t c = next_pow2_u32((uint32_t)(2 * np));
@@ -690,7 +695,7 @@ __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const Ite
                                    uint64_t* masks,
                                    const int32_t* pstage, int np, int lane, int wave) {
   if (hq < 0) {
-    const uint32_t cap = table_cap_lds(np);
+    const uint32_t cap = EV_LDS_CAP;
     int poff[EV_MAXK];
 #pragma unroll
     for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
@@ -893,7 +898,7 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
           for (int i = threadIdx.x; i < np; i += EV_THREADS) pstage[c * np + i] = src[i];
         }
       }
-      for (uint32_t i = threadIdx.x, cap = table_cap_lds(np); i < cap; i += EV_THREADS) tbl[i] = 0;
+      clear_lds_table(tbl);
     }
     // Start loading the next unit's facts; they arrive while this unit is evaluated.
     {
@@ -909,7 +914,7 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
 
     // Join phase (subset 0: the join column alone).
     if (hq < 0) {
-      const uint32_t cap = table_cap_lds(np);
+      const uint32_t cap = EV_LDS_CAP;
       const int poff = U->jcol * np;
       build_lds_table<1>(tbl, cap, pstage, &poff, np, 1);
       __syncthreads();  // table ready
@@ -946,7 +951,7 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
         }
       }
       if (hq < 0)
-        for (uint32_t i = threadIdx.x, cap = table_cap_lds(np); i < cap; i += EV_THREADS) tbl[i] = 0;
+        clear_lds_table(tbl);
       __syncthreads();  // entry list, need flag and cleared table ready
       if (uni(sh.ent_start[nseg]) == 0 || uni(sh.need[s]) == 0) continue;
       const int nk = U->sub[s].nk;
@@ -1101,8 +1106,7 @@ static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table&
   std::vector<int32_t> hq(parts, -1);
   for (int64_t q = 0; q < parts; ++q) {
     const int64_t np = pt->h_offsets[q + 1] - pt->h_offsets[q];
-    if (np > 0 && (np > EV_LDS_MAX_FACTS || (int64_t)table_cap_lds(np) > EV_TBL_WORDS ||
-                   np * rel.num_columns > EV_PST_WORDS)) {
+    if (np > 0 && (np > EV_LDS_MAX_FACTS || np * rel.num_columns > EV_PST_WORDS)) {
       hq[q] = (int32_t)rp->heavy_q.size();
       rp->heavy_q.push_back(q);
     }
