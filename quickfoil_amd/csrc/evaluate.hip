@@ -387,19 +387,23 @@ __device__ inline uint32_t lds_probe(const uint32_t* tbl, uint32_t cap, const in
   const uint32_t h = lds_hash(vals, NK ? NK : nk);
   const uint32_t fp = lds_fp(h);
   uint32_t slot = lds_slot(h, cap);
-  while (true) {
+  uint32_t out = 0;
+  bool live = true;
+  // One exit test per slot; the key check runs only on a fingerprint match.
+  while (live) {
     const uint32_t w = tbl[slot];
-    if (w == 0) return 0;
-    if ((w & EV_FP_MASK) == fp) {
+    bool eq = w != 0 && (w & EV_FP_MASK) == fp;
+    if (eq) {
       const int r = (int)((w >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1;
-      bool eq = true;
 #pragma unroll
       for (int k = 0; k < (NK ? NK : EV_MAXK); ++k)
         if (NK || k < nk) eq &= (pstage[poff[k] + r] == vals[k]);
-      if (eq) return w & EV_FIELD_MASK;
+      if (eq) out = w & EV_FIELD_MASK;
     }
+    live = w != 0 && !eq;
     slot = (slot + 1) & (cap - 1);
   }
+  return out;
 }
 
 // Wave-uniform registers of the literal being probed.  Its row-side equalities (rare: only a
