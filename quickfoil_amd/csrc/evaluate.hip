@@ -160,8 +160,9 @@ __device__ inline void clear_lds_table(uint32_t* tbl) {
   for (int i = threadIdx.x; i < (int)EV_LDS_CAP / 4; i += QF_EV_THREADS)
     reinterpret_cast<uint4*>(tbl)[i] = make_uint4(0u, 0u, 0u, 0u);
 }
+// HBM tables: cap = P2(4 * facts) slots (load <= 1/4), P2(2 * facts) above 2^29 facts.
 __host__ __device__ inline uint32_t table_cap_hbm(int64_t np) {
-  uint32_t c = next_pow2_u32((uint32_t)(2 * np));
+  uint32_t c = next_pow2_u32((uint32_t)(np < (1ll << 29) ? 4 * np : 2 * np));
   return c < EV_MIN_CAP ? EV_MIN_CAP : c;
 }
 
@@ -181,14 +182,18 @@ __device__ inline uint32_t table_claim(uint32_t* state, uint32_t cap, const int3
 __device__ inline uint32_t table_probe(const uint32_t* cnt, const int32_t* keys, uint32_t cap, const int32_t* vals,
                                        int nk) {
   uint32_t slot = tuple_hash(vals, nk) & (cap - 1);
-  while (true) {
+  uint32_t out = 0;
+  bool live = true;
+  // A slot's count and keys are loaded together (one memory round trip per slot).
+  while (live) {
     const uint32_t c = cnt[slot];
-    if (c == 0) return 0;
-    bool eq = true;
-    for (int k = 0; k < nk; ++k) eq = eq && (keys[(size_t)k * cap + slot] == vals[k]);
-    if (eq) return c;
+    bool eq = c != 0;
+    for (int k = 0; k < nk; ++k) eq &= keys[(size_t)k * cap + slot] == vals[k];
+    if (eq) out = c;
+    live = c != 0 && !eq;
     slot = (slot + 1) & (cap - 1);
   }
+  return out;
 }
 
 __device__ inline uint64_t wave_sum_u64(uint64_t v) {
