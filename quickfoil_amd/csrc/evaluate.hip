@@ -47,7 +47,7 @@ constexpr int EV_THREADS = QF_EV_THREADS;  // threads of one evaluation block (a
 constexpr int EV_WAVES = EV_THREADS / 64;
 constexpr int EV_MAXK = QF_MAX_ARITY;  // key columns of one subset tuple (distinct relation columns)
 constexpr int EV_MAXEQ = 4;            // binding-side equalities of one literal (repeated columns)
-constexpr int EV_MAXSUB = 32;          // key subsets per unit
+constexpr int EV_MAXSUB = 32;          // key subsets per unit (bits of a uint32)
 constexpr int EV_MAXTC = 12;           // staged binding columns (plus the row id)
 constexpr int EV_MAXG = EV_MAXTC;      // join variables per unit
 constexpr int EV_MAXLU = 512;          // literals per unit
@@ -91,8 +91,7 @@ struct DevUnit {
   int nsub;
   int jcol;               // the join column (subset 0's key)
   uint32_t orphan;        // bit s: subset s has a literal without a parent mask
-  uint64_t parmask[EV_MAXSUB];  // subset s: the mask slots of its literals' parents
-  int8_t slot_group[EV_MAXMASK];  // the unit group of the literal owning each mask slot
+  uint32_t child_subsets[EV_MAXMASK];  // per mask slot: the subsets of the literals it prunes
   int ug_of_bg[EV_MAXG];  // the unit's group for each group of its batch, or -1
   DevUGroup g[EV_MAXG];
   DevSubset sub[EV_MAXSUB];
@@ -375,8 +374,7 @@ struct ItemShared {
   int seg_hits[EV_MAXG];       // rows of the segment whose join value occurs in the partition
   int blk_start[EV_MAXG + 1];  // join phase: 64-row blocks of all rows, per segment
   int ent_start[EV_MAXG + 1];  // subset phase: (literal, block of hit rows) entries, per segment
-  int need[EV_MAXSUB];         // subset s has a literal whose parent mask is not all zero
-  int seg_of_ug[EV_MAXG];      // the current unit's segment for each of its groups, or -1
+  uint32_t need;               // bit s: a parent of a literal of subset s matched a row of this item
   int64_t u_p0[EV_MAXBU];      // per unit of the batch: first fact of cell q, fact count, heavy ordinal
   int u_np[EV_MAXBU];
   int u_hq[EV_MAXBU];
@@ -583,7 +581,7 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
 // with its own mask slot records its hit rows for its children.  kLds: packed LDS table `tbl`;
 // else the HBM table (cnt, keys).  NK > 0 fixes the key width; NK == 0 reads it from `nk`.
 template <int NK, bool kLds>
-__device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk,
+__device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, int s, int nk,
                               const int32_t* tstage, const uint8_t* posf, const uint16_t* cidx, uint64_t* masks, const uint32_t* tbl,
                               const uint32_t* cnt, const int32_t* keys, uint32_t cap, const int32_t* pstage, int np,
                               int lane, int wave) {
@@ -644,7 +642,10 @@ __device__ void probe_entries(const EvalArgs& a, cptr<DevUnit> U, const ItemShar
     }
     if (L.mslot >= 0) {
       const uint64_t hm = __ballot(c != 0);
-      if (lane == 0) masks[L.mslot * mstride + blk] = hm;
+      if (lane == 0) {
+        masks[L.mslot * mstride + blk] = hm;
+        if (hm != 0) atomicOr(&sh.need, U->child_subsets[L.mslot]);
+      }
     }
     if (++blk == nblk) {  // the literal's run in this segment ends
       blk = 0;
@@ -699,7 +700,7 @@ __device__ void build_lds_table(uint32_t* tbl, uint32_t cap, const int32_t* psta
 // One key subset s >= 1 of an item: build its count table (LDS partitions; the table words were
 // cleared before the barrier that precedes this call), then probe.
 template <int NK>
-__device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, const ItemShared& sh, int s, int nk, int hq,
+__device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, int s, int nk, int hq,
                                    uint32_t* tbl, const int32_t* tstage, const uint8_t* posf, const uint16_t* cidx,
                                    uint64_t* masks,
                                    const int32_t* pstage, int np, int lane, int wave) {
@@ -882,9 +883,7 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
       const int bs = wave_excl_prefix(blocks, lane);
       if (ok) sh.blk_start[lane] = bs;
       if (lane == 63) sh.blk_start[nseg] = bs + blocks;
-      if (lane < EV_MAXSUB) sh.need[lane] = 0;
-      if (lane < EV_MAXG) sh.seg_of_ug[lane] = -1;
-      if (ok && ug >= 0) sh.seg_of_ug[ug] = lane;
+      if (lane == 0) sh.need = 0;
     }
     // The partition's facts (all relation columns) for an LDS table, from the registers the
     // previous unit prefetched them into when it could.
@@ -936,7 +935,10 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
     }
 
     for (int s = 1; s < U->nsub; ++s) {
-      __syncthreads();  // hit lists / previous subset's probes and masks done
+      __syncthreads();  // hit lists / previous subset's probes, masks and need bits done
+      // Whether any literal of the subset can match: one without a parent, or one whose parent
+      // matched a row of this item (a parent's subset has fewer columns, so it ran earlier).
+      if (!(((U->orphan | uni((int)sh.need)) >> s) & 1u)) continue;
       if (wave == 0) {  // entry list, one lane per segment
         const bool ok = lane < nseg;
         const int g = ok ? sh.seg_ug[lane] : -1;
@@ -945,24 +947,10 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
         if (ok) sh.ent_start[lane] = es;
         if (lane == 63) sh.ent_start[nseg] = es + e;
       }
-      // Whether any literal of the subset can match: one without a parent mask, or a parent
-      // mask with a row set (masks left by earlier items only add work, never skip it).
-      if ((U->orphan >> s) & 1u) {
-        if (threadIdx.x == 0) sh.need[s] = 1;
-      } else {
-        // Each parent's mask words of its segment's blocks of listed rows.
-        const uint64_t pm = U->parmask[s];
-        for (int i = threadIdx.x; i < EV_MAXMASK * a.mstride; i += EV_THREADS) {
-          const int slot = i / a.mstride, b = i - slot * a.mstride;
-          if (!((pm >> slot) & 1ull)) continue;
-          const int sg = sh.seg_of_ug[U->slot_group[slot]];
-          if (sg >= 0 && b < ((sh.seg_hits[sg] + 63) >> 6) && masks[i] != 0) sh.need[s] = 1;
-        }
-      }
       if (hq < 0)
         clear_lds_table(tbl);
-      __syncthreads();  // entry list, need flag and cleared table ready
-      if (uni(sh.ent_start[nseg]) == 0 || uni(sh.need[s]) == 0) continue;
+      __syncthreads();  // entry list and cleared table ready
+      if (uni(sh.ent_start[nseg]) == 0) continue;
       const int nk = U->sub[s].nk;
       switch (nk) {
         case 1: eval_subset<1>(a, U, sh, s, nk, hq, tbl, tstage, posf, cidx, masks, pstage, np, lane, wave); break;
@@ -1341,7 +1329,6 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
                      [](const SubKey& x, const SubKey& y) { return x.first.size() < y.first.size(); });
     QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one unit");
     du.jcol = jc;
-    for (int m = 0; m < EV_MAXMASK; ++m) du.slot_group[m] = -1;
     int unit_lits = 0, nmask = 0;
     const size_t unit_first = P.lits.size();
     for (int gi = 0; gi < du.ngroups; ++gi) {
@@ -1393,7 +1380,6 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
           DevLit& pd = P.lits[first + j];
           if (pd.mslot < 0) {
             if (nmask >= EV_MAXMASK) break;
-            du.slot_group[nmask] = (int8_t)gi;
             pd.mslot = nmask++;
           }
           P.lits[first + i].pmask = pd.mslot;
@@ -1403,11 +1389,11 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     QF_REQUIRE(unit_lits <= EV_MAXLU, "too many literals in one unit");
     P.max_mask = std::max(P.max_mask, nmask);
     du.orphan = 0;
-    for (int s2 = 0; s2 < EV_MAXSUB; ++s2) du.parmask[s2] = 0;
+    for (int m = 0; m < EV_MAXMASK; ++m) du.child_subsets[m] = 0;
     for (size_t i = unit_first; i < P.lits.size(); ++i) {
       const DevLit& dl = P.lits[i];
       if (dl.pmask < 0) du.orphan |= 1u << dl.subset;
-      else du.parmask[dl.subset] |= 1ull << dl.pmask;
+      else du.child_subsets[dl.pmask] |= 1u << dl.subset;
     }
     du.nsub = (int)subsets.size();
     for (int s = 0; s < du.nsub; ++s) {
