@@ -533,7 +533,7 @@ __device__ void join_phase(const EvalArgs& a, cptr<DevUnit> U, ItemShared& sh, c
     if (ma | mb) {
       int base = 0;
       if (lane == 0) base = atomicAdd(&sh.seg_hits[sg], __popcll(ma) + __popcll(mb));
-      base = __shfl(base, 0, 64);
+      base = __builtin_amdgcn_readlane(base, 0);
       if (ca != 0) cidx[seg_off + base + (int)lane_prefix(ma, lane)] = (uint16_t)ra;
       if (cb != 0) cidx[seg_off + base + __popcll(ma) + (int)lane_prefix(mb, lane)] = (uint16_t)rb;
       if (l1 > l0) {
@@ -722,21 +722,35 @@ __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, ItemShare
   }
 }
 
-// Exclusive prefix sums across the 64 lanes of a wave.
-__device__ inline int wave_excl_prefix(int v, int lane) {
+// Exclusive prefix sums over lanes 0-15, the segments of an item (at most EV_MAXG of them), by
+// DPP shifts within the wave's first row of 16 lanes (row_shr 1, 2, 4, 8: no LDS round trips);
+// lane 15 ends with its exclusive sum, so lane 15's value plus its own is the total.  Lanes 16-63
+// hold sums over their own rows, which callers ignore.  Every lane of the wave must be active.
+static_assert(EV_MAXG <= 16, "segments fit one DPP row");
+template <int kShr>
+__device__ inline int32_t row_shr(int32_t x) {
+  return __builtin_amdgcn_update_dpp(0, x, 0x110 + kShr, 0xf, 0xf, false);  // lanes shifted in read 0
+}
+__device__ inline int row_excl_prefix(int v) {
   int x = v;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
+  x += row_shr<1>(x);
+  x += row_shr<2>(x);
+  x += row_shr<4>(x);
+  x += row_shr<8>(x);
   return x - v;
 }
-__device__ inline int64_t wave_excl_prefix64(int64_t v, int lane) {
+template <int kShr>
+__device__ inline int64_t row_shr64(int64_t x) {
+  const uint32_t lo = (uint32_t)row_shr<kShr>((int32_t)(uint32_t)x);
+  const uint32_t hi = (uint32_t)row_shr<kShr>((int32_t)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ inline int64_t row_excl_prefix64(int64_t v) {
   int64_t x = v;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int64_t y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
+  x += row_shr64<1>(x);
+  x += row_shr64<2>(x);
+  x += row_shr64<4>(x);
+  x += row_shr64<8>(x);
   return x - v;
 }
 
@@ -814,12 +828,12 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
         t0 = B->toff[g][q << B->shift];
         t1 = B->toff[g][(q + 1) << B->shift];
       }
-      const int64_t run = wave_excl_prefix64(t1 - t0, lane);
+      const int64_t run = row_excl_prefix64(t1 - t0);
       const int64_t lo_ = max(run, c0), hi_ = min(run + (t1 - t0), c1);
       const int n = (ok && hi_ > lo_) ? (int)(hi_ - lo_) : 0;
       const uint64_t has = __ballot(n > 0);
       const int idx = __popcll(has & ((1ull << lane) - 1ull));
-      const int off = wave_excl_prefix(n, lane);
+      const int off = row_excl_prefix(n);
       if (n > 0) {
         sh.seg_group[idx] = g;
         sh.seg_off[idx] = off;
@@ -827,7 +841,7 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
         sh.seg_src[idx] = t0 + (lo_ - run);
         sh.seg_tid[idx] = B->tcols[g][ntc] + (t0 + (lo_ - run));
       }
-      if (lane == 63) {
+      if (lane == 15) {
         sh.seg_off[__popcll(has)] = off + n;
         sh.nseg = __popcll(has);
       }
@@ -880,9 +894,9 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
         sh.seg_ug[lane] = ug;
         sh.seg_hits[lane] = 0;
       }
-      const int bs = wave_excl_prefix(blocks, lane);
+      const int bs = row_excl_prefix(blocks);
       if (ok) sh.blk_start[lane] = bs;
-      if (lane == 63) sh.blk_start[nseg] = bs + blocks;
+      if (lane == 15) sh.blk_start[nseg] = bs + blocks;
       if (lane == 0) sh.need = 0;
     }
     // The partition's facts (all relation columns) for an LDS table, from the registers the
@@ -943,9 +957,9 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
         const bool ok = lane < nseg;
         const int g = ok ? sh.seg_ug[lane] : -1;
         const int e = g >= 0 ? (U->g[g].sbeg[s + 1] - U->g[g].sbeg[s]) * ((sh.seg_hits[lane] + 63) >> 6) : 0;
-        const int es = wave_excl_prefix(e, lane);
+        const int es = row_excl_prefix(e);
         if (ok) sh.ent_start[lane] = es;
-        if (lane == 63) sh.ent_start[nseg] = es + e;
+        if (lane == 15) sh.ent_start[nseg] = es + e;
       }
       if (hq < 0)
         clear_lds_table(tbl);
