@@ -122,6 +122,26 @@ def test_microbench_shape_scaled(qf_ctx, dist):
         np.testing.assert_array_equal(bits, want_bits)
 
 
+@pytest.mark.parametrize("partition_rows,tile,semi", [(0, 0, False), (128, 512, False), (256, 768, True),
+                                                     (512, 2048, False)])
+def test_refinement_step_scaled_golden(qf_ctx, partition_rows, tile, semi):
+    """The refinement-step workload at 1/256 scale (2,080 literals over 20 relations of arity 2 and
+    3: key subsets of two and three columns, whose parents' hits gate them) against the oracle's
+    committed counts; covered bitvectors, where asked for, popcount to the covered counts."""
+    import os
+    from quickfoil_amd import datagen
+    want = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "step_1_256_counts.npy"))
+    w = datagen.refinement_step(scale=1 / 256)
+    got, bits = _gpu_eval(qf_ctx, w["binding"], w["num_positive"], w["relations"], w["literals"],
+                          partition_rows=partition_rows, tile_rows=tile, semi_bits=semi)
+    np.testing.assert_array_equal(got, want)
+    if semi:
+        n, npos = len(w["binding"][0]), w["num_positive"]
+        rows = np.unpackbits(bits.view(np.uint8), axis=1, bitorder="little")[:, :n]
+        np.testing.assert_array_equal(rows[:, :npos].sum(axis=1), want[:, 0])
+        np.testing.assert_array_equal(rows[:, npos:].sum(axis=1), want[:, 1])
+
+
 def test_graph_replay_matches_eager(qf_ctx):
     binding, npos, rels, lits = random_problem(41, n=30000, rel_shapes=((2, 20000), (3, 9000)), domain=300,
                                                zipf=True, max_literals=60)

@@ -113,6 +113,18 @@ def test_pipeline_uncle_sample_data():
     assert np.all(pc[:, 0] <= npos) and np.all(pc[:, 1] <= len(target[0]) - npos)
 
 
+def test_step_golden_fixture_matches_pipeline():
+    """The committed scaled-step counts (written by the grouped-count closed form) equal the
+    oracle's restatement of the reference's materialising pipeline on the same workload."""
+    import os
+    from quickfoil_amd import datagen
+    want = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "step_1_256_counts.npy"))
+    w = datagen.refinement_step(scale=1 / 256)
+    got, _, _ = oracle.Problem(w["binding"], w["num_positive"], w["relations"], w["literals"]).pipeline()
+    assert want.shape == (2080, 4) and int(want.sum()) > 0
+    np.testing.assert_array_equal(got, want)
+
+
 def test_counts_hand_example():
     # T = [(1,2),(1,3),(2,2)] with row 0 positive; P = {(1,2),(1,2),(1,5),(2,9)}.
     T = [np.array([1, 1, 2], np.int32), np.array([2, 3, 2], np.int32)]
