@@ -11,9 +11,10 @@
 //
 // Layout.  Relations and the binding table are co-partitioned by the top bits of mix32(join
 // value): the binding table once per clause join variable with R_T bits, carrying the referenced
-// columns and the row id; each relation on each join column with its own R_u <= R_T bits
-// (chosen from its size), carrying all its columns.  Top bits nest, so relation partition q
-// covers the contiguous binding partitions [q << (R_T - R_u), (q + 1) << (R_T - R_u)).
+// columns and the row id; each relation on each join column with R_u <= R_T bits (chosen from
+// its size; the batched form gives every relation R_T), carrying all its columns.  Top bits
+// nest, so relation partition q covers the binding partitions [q << (R_T - R_u), (q + 1) <<
+// (R_T - R_u)).
 //
 // Work.  A unit is (relation, join column) with the literal groups of every join variable that
 // uses it (one literal for the plain form).  Units with the same radix bits form a batch: they
@@ -1252,6 +1253,11 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     unit_R[std::make_pair(g.relation, g.join_col)] = R;
     P.R_T = std::max(P.R_T, R);
   }
+  // Batched: every relation takes the binding's R_T bits.  Units of all sizes then share batches
+  // (a batch stages the binding rows once, so fewer distinct R means fewer passes over them), and
+  // binding rows skip a small relation's empty partitions instead of probing its table.
+  if (o.mode == QF_EVAL_BATCHED && o.radix_bits <= 0)
+    for (auto& kv : unit_R) kv.second = P.R_T;
   std::set<int> bset, vset;
   for (const GroupSpec& g : specs) {
     vset.insert(g.join_var);
