@@ -56,7 +56,7 @@ constexpr int EV_TBL_WORDS = 2048;     // LDS words for one count table (8 KB)
 constexpr int EV_PST_WORDS = 1536;     // LDS words for the partition's facts (6 KB)
 constexpr int EV_MIN_CAP = 16;
 constexpr int EV_MAXMASK = 64;         // literals per unit whose hit masks prune other literals
-constexpr int EV_MAXBU = 16;           // units per batch (units of equal radix bits share staged rows)
+constexpr int EV_MAXBU = 32;           // units per batch (units of equal radix bits share staged rows)
 
 struct DevSubset {
   int nk;
