@@ -171,7 +171,10 @@ __device__ inline uint32_t table_claim(uint32_t* state, uint32_t cap, const int3
                                        const int32_t* const* pcols, const int* pcol, int64_t p0, int64_t r) {
   uint32_t slot = tuple_hash(vals, nk) & (cap - 1);
   while (true) {
-    const uint32_t prev = atomicCAS(&state[slot], 0u, (uint32_t)(r + 1));
+    // A claimed slot never changes, so a plain read settles most rows of a skewed partition
+    // (which repeat one key) without queueing an atomic on that slot; an empty read is claimed.
+    uint32_t prev = __hip_atomic_load(&state[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == 0) prev = atomicCAS(&state[slot], 0u, (uint32_t)(r + 1));
     if (prev == 0) return slot;
     bool eq = true;
     for (int k = 0; k < nk; ++k) eq = eq && (pcols[pcol[k]][p0 + prev - 1] == vals[k]);
