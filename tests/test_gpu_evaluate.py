@@ -142,6 +142,62 @@ def test_refinement_step_scaled_golden(qf_ctx, partition_rows, tile, semi):
         np.testing.assert_array_equal(rows[:, npos:].sum(axis=1), want[:, 1])
 
 
+def _covered_from_bits(bits, n, npos):
+    rows = np.unpackbits(bits.view(np.uint8), axis=1, bitorder="little")[:, :n]
+    return rows[:, :npos].sum(axis=1), rows[:, npos:].sum(axis=1)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "zipf"])
+def test_microbench_full_size(qf_ctx, dist):
+    """The operator microbench at full size (16,777,216 x 3 bindings, 4,194,304 x 2 facts over 1 M
+    keys) against the oracle: its materialising pipeline for uniform keys, the grouped-count closed
+    form for Zipf keys (5.6e11 matching pairs, beyond what a pipeline can hold); the covered
+    bitvectors popcount to the covered counts."""
+    from quickfoil_amd import datagen
+    w = datagen.microbench(dist)
+    prob = oracle.Problem(w["binding"], w["num_positive"], w["relations"], w["literals"])
+    want = prob.pipeline()[0] if dist == "uniform" else prob.grouped()[0]
+    got, bits = _gpu_eval(qf_ctx, w["binding"], w["num_positive"], w["relations"], w["literals"], semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    cov_pos, cov_neg = _covered_from_bits(bits, len(w["binding"][0]), w["num_positive"])
+    np.testing.assert_array_equal(cov_pos, want[:, 0])
+    np.testing.assert_array_equal(cov_neg, want[:, 1])
+    qf_ctx.trim()
+
+
+def test_refinement_step_full_size_batched_equals_plain(qf_ctx):
+    """The refinement step at full size (8,388,608 x 4 bindings, 20 relations of up to 26.7 M facts,
+    2,080 literals), where no CPU oracle finishes in test time: the batched form (one radix-bit
+    count for every relation, batches of up to 32 units, graph replay) equals the plain per-group
+    form (each relation at its own radix bits, one launch per group), and the counts obey
+    covered <= binding, covered positives <= positives, and monotonicity: binding one more
+    position of a literal never raises any of its four counts."""
+    import torch
+    from quickfoil_amd import datagen
+    w = datagen.refinement_step(device=torch.device("cuda:0"))
+    args = (w["binding"], w["num_positive"], w["relations"], w["literals"])
+    batched, _ = qf_ctx.evaluate_literals(*args, reuse_partitions=True, use_graph=True)
+    plain, _ = qf_ctx.evaluate_literals(*args, reuse_partitions=True, mode="plain_group")
+    np.testing.assert_array_equal(batched, plain)
+    c = batched.astype(np.int64)
+    n, npos = int(w["binding"][0].numel()), w["num_positive"]
+    assert (c[:, 0] <= c[:, 2]).all() and (c[:, 1] <= c[:, 3]).all()
+    assert (c[:, 0] <= npos).all() and (c[:, 1] <= n - npos).all()
+    assert c.sum() > 0
+    index = {(r, tuple(v)): i for i, (r, v) in enumerate(w["literals"])}
+    pairs = 0
+    for i, (r, v) in enumerate(w["literals"]):
+        for p, x in enumerate(v):
+            if x >= 0:
+                continue
+            for var in range(len(w["binding"])):
+                j = index[(r, tuple(v[:p]) + (var,) + tuple(v[p + 1:]))]
+                assert (c[j] <= c[i]).all(), (w["literals"][i], w["literals"][j])
+                pairs += 1
+    assert pairs > 1000
+    qf_ctx.trim()
+
+
 def test_graph_replay_matches_eager(qf_ctx):
     binding, npos, rels, lits = random_problem(41, n=30000, rel_shapes=((2, 20000), (3, 9000)), domain=300,
                                                zipf=True, max_literals=60)
