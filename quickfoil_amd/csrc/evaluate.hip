@@ -30,9 +30,9 @@
 // the (literal, 64-row block of listed rows) entries of that subset literal-major.  A literal of
 // three or more key columns skips the blocks where its parent (its key minus one column) matched
 // nothing, and a subset none of whose literals can match (no orphan, no parent hit in the item)
-// is skipped whole.  Covered counts come from ballots; binding counts from per-lane sums reduced once per
-// literal run and added by four lanes into the workgroup's copy of the counters, which are
-// summed after the launches.
+// is skipped whole.  Covered counts come from ballots; binding counts from per-lane sums reduced
+// once per literal run and added by four lanes into the workgroup's copy of the counters, which
+// are summed after the launches.
 #include "qf_internal.h"
 
 #include <algorithm>
