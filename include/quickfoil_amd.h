@@ -117,7 +117,8 @@ typedef enum {
 
 typedef struct {
   int mode;              /* qf_eval_mode */
-  int radix_bits;        /* 0 = autotuned; else partition bits for both sides */
+  int radix_bits;        /* 0 = from partition_rows; else partition bits (at most 16) for both
+                            sides */
   int tile_rows;         /* 0 = autotuned; binding rows per work item */
   int use_graph;         /* batched mode: capture the step's device work as a hipGraph and
                             replay it when the same call repeats (same columns, literals,
@@ -128,7 +129,8 @@ typedef struct {
                             291-299).  Only for relations that stay unchanged at that address
                             until qf_context_trim; default 0. */
   int partition_rows;    /* 0 = default (256) or autotuned; target facts per relation partition,
-                            from which each relation's radix bits follow */
+                            from which each relation's radix bits follow (at most 16); the
+                            batched mode then gives every relation the largest of them */
   int autotune;          /* 1: on the first call for a workload shape, time a grid of
                             (partition_rows, tile_rows) and keep the fastest in the context */
 } qf_eval_options;
