@@ -1074,7 +1074,8 @@ struct TuneCache {
 // HBM count tables of the heavy partitions of one (relation, join column) for one subset.
 struct HeavyTables {
   DevBuf buf;
-  DevBuf base;  // int64 per heavy ordinal
+  DevBuf base;  // int64 per heavy ordinal, then the build and finalize job lists (kept with the
+                // table, so that nothing waits for the build kernels)
 };
 
 // A relation partitioned on one join column, with its heavy partitions and their tables.
@@ -1151,13 +1152,19 @@ static std::shared_ptr<HeavyTables> heavy_tables(Context* ctx, RelPart& rp, cons
     words += (int64_t)cap * (nk + 2);
   }
   ht->buf = DevBuf(ctx, std::max<int64_t>(1, words) * 4);
-  ht->base = DevBuf(ctx, std::max<size_t>(1, base.size()) * 8);
+  static_assert(sizeof(HeavyJob) % 8 == 0, "job lists follow the int64 offsets");
+  const size_t base_bytes = std::max<size_t>(1, base.size()) * 8;
+  ht->base = DevBuf(ctx, base_bytes + (build.size() + fin.size()) * sizeof(HeavyJob));
   if (!base.empty()) {
     PhaseScope ps(ctx, "heavy_tables");
-    copy_h2d(ctx, ht->base.get(), base.data(), base.size() * 8);
-    DevBuf dbuild(ctx, build.size() * sizeof(HeavyJob)), dfin(ctx, fin.size() * sizeof(HeavyJob));
-    copy_h2d(ctx, dbuild.get(), build.data(), build.size() * sizeof(HeavyJob));
-    copy_h2d(ctx, dfin.get(), fin.data(), fin.size() * sizeof(HeavyJob));
+    // Offsets and both job lists in one upload.
+    std::vector<uint8_t> host(base_bytes + (build.size() + fin.size()) * sizeof(HeavyJob));
+    std::memcpy(host.data(), base.data(), base.size() * 8);
+    std::memcpy(host.data() + base_bytes, build.data(), build.size() * sizeof(HeavyJob));
+    std::memcpy(host.data() + base_bytes + build.size() * sizeof(HeavyJob), fin.data(), fin.size() * sizeof(HeavyJob));
+    copy_h2d(ctx, ht->base.get(), host.data(), host.size());
+    const HeavyJob* dbuild = reinterpret_cast<const HeavyJob*>(ht->base.as<uint8_t>() + base_bytes);
+    const HeavyJob* dfin = dbuild + build.size();
     fill_i32(ctx, ht->buf.as<int32_t>(), 0, words);
     DevRel rel{};
     for (int c = 0; c < rp.pt->num_columns; ++c) rel.pcols[c] = rp.pt->columns[c].as<int32_t>();
@@ -1166,12 +1173,11 @@ static std::shared_ptr<HeavyTables> heavy_tables(Context* ctx, RelPart& rp, cons
     sub.nk = nk;
     for (int k = 0; k < nk; ++k) sub.pcol[k] = pcols[k];
     hipLaunchKernelGGL(k_heavy_build, dim3((unsigned)build.size()), dim3(EV_THREADS), 0, ctx->stream, rel, sub,
-                       dbuild.as<HeavyJob>(), ht->buf.as<uint32_t>());
+                       dbuild, ht->buf.as<uint32_t>());
     QF_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_heavy_finalize, dim3((unsigned)fin.size()), dim3(EV_THREADS), 0, ctx->stream, rel, sub,
-                       dfin.as<HeavyJob>(), ht->buf.as<uint32_t>());
+                       dfin, ht->buf.as<uint32_t>());
     QF_CHECK_HIP(hipGetLastError());
-    sync(ctx);  // job lists are released at the end of this scope
   }
   rp.tables[pcols] = ht;
   return ht;
