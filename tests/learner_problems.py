@@ -1,5 +1,5 @@
 """Learning problems shared by the learner tests: the reference's sample data (uncle, Bongard
-th3/100 fold 0) and the synthetic kinship and Bongard generators."""
+th3/100 and th3/1000 fold 0) and the synthetic kinship and Bongard generators."""
 import ctypes
 import os
 import sys
@@ -22,7 +22,8 @@ def uncle():
     return learner.Problem(rel("uncle"), 4, [rel("brother"), rel("parent")]), {}
 
 
-# sample_data/bongard/th3/100/fold_0/bongard.json:4-126: relation name, file, domain types.
+# sample_data/bongard/th3/100/fold_0/bongard.json:4-126 (th3/1000/fold_0/bongard.json:1-118 lists
+# the same relations and types): relation name, file, domain types.
 _BONGARD = [
     ("element_train", "element_merge", [0, 1]),
     ("circle_train", "circle_merge", [1]),
@@ -36,11 +37,21 @@ _BONGARD = [
 ]
 
 
-def bongard_sample():
-    d = os.path.join(GOLDEN, "bongard100")
+# Positives of each fold's training target and test file: th3/100/fold_0/bongard.json:15-25
+# (test num_positive 10, 88 training positives) and th3/1000/fold_0/bongard.json:14-21 (100; 901).
+_BONGARD_FOLDS = {"bongard100": (88, 10), "bongard1000": (901, 100)}
+
+
+def bongard_sample(fold="bongard100"):
+    d = os.path.join(GOLDEN, fold)
     bg = [learner.Relation(n, load_relation(os.path.join(d, f), len(t)), t) for n, f, t in _BONGARD]
     target = learner.Relation("bongard", load_relation(os.path.join(d, "bongard_train_5"), 1), [0])
-    return learner.Problem(target, 88, bg), {}
+    return learner.Problem(target, _BONGARD_FOLDS[fold][0], bg), {}
+
+
+def bongard_test_file(fold):
+    """The fold's test examples (positives first) and their number of positives."""
+    return load_relation(os.path.join(GOLDEN, fold, "bongard_test"), 1), _BONGARD_FOLDS[fold][1]
 
 
 def kinship(num_persons=2000):
@@ -80,5 +91,7 @@ PROBLEMS_BY_NAME = {
     "uncle": uncle,
     "bongard_sample": bongard_sample,
     "kinship": kinship,
+    "bongard_sample_1000": lambda: bongard_sample("bongard1000"),
     "bongard_synthetic": lambda: bongard_synthetic(5000),
+    "bongard_synthetic_200k": lambda: bongard_synthetic(200000),
 }

@@ -213,3 +213,59 @@ def test_graph_replay_matches_eager(qf_ctx):
         again, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, lits, mode="plain_group")
         np.testing.assert_array_equal(again, want)
     qf_ctx.trim()
+
+
+def test_autotuned_step_matches_golden(qf_ctx):
+    """autotune=1 times its (facts per partition, tile) grid on the first call for a workload shape
+    and runs the fastest; the counts of the 1/256 refinement step equal the committed golden ones,
+    on the tuning call and on the cached choice."""
+    import os
+    from quickfoil_amd import datagen
+    want = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "step_1_256_counts.npy"))
+    w = datagen.refinement_step(scale=1 / 256)
+    b = [_t(c) for c in w["binding"]]
+    r = [[_t(c) for c in rel] for rel in w["relations"]]
+    for _ in range(2):
+        got, _ = qf_ctx.evaluate_literals(b, w["num_positive"], r, w["literals"], autotune=True, reuse_partitions=True)
+        np.testing.assert_array_equal(got, want)
+        choice = qf_ctx.autotune_choice()
+        assert choice["partition_rows"] in (128, 256, 512)
+        assert choice["tile_rows"] in (512, 768, 1024, 1536, 2048)
+        assert choice["best_ms"] > 0
+    qf_ctx.trim()
+
+
+def test_count_range_is_reported_with_exact_counts(qf_ctx):
+    """A binding count above int32 (60,000 positive rows x 40,000 facts of one join value: 2.4e9
+    pairs) returns QF_ERR_COUNT_RANGE when asked for, and the uint64 counts are exact either way (the
+    reference's int32 size_type would wrap there, learner/CandidateLiteralInfo.hpp:21-25)."""
+    from quickfoil_amd._native import QfError
+    n, m, npos = 70000, 40000, 60000
+    binding = [np.zeros(n, np.int32), np.arange(n, dtype=np.int32)]
+    rel = [[np.zeros(m, np.int32), np.arange(m, dtype=np.int32) + 5]]
+    lits = [(0, [0, -1]), (0, [0, 1])]
+    want, _ = oracle.Problem(binding, npos, rel, lits).grouped()
+    assert int(want[0][2]) == npos * m > 2 ** 31 and int(want[0][3]) == (n - npos) * m
+    got, _ = _gpu_eval(qf_ctx, binding, npos, rel, lits)
+    np.testing.assert_array_equal(got, want)
+    with pytest.raises(QfError) as e:
+        _gpu_eval(qf_ctx, binding, npos, rel, lits, check_range=True)
+    assert e.value.status == 5
+    qf_ctx.trim()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_arg_types_join_key_rule(qf_ctx, mode):
+    """Argument types choose each literal's join key (schema/FoilLiteral.hpp:133-144: the bound
+    position with the largest type id, first on ties), which regroups the literals; the counts are
+    those of the brute-force definition either way."""
+    binding, npos, rels, lits = random_problem(31, n=500, rel_shapes=((2, 300), (3, 250)), domain=14)
+    arg_types = [[0, 2], [1, 0, 3]]
+    prob = oracle.Problem(binding, npos, rels, lits, arg_types=arg_types)
+    want, want_bits = prob.bruteforce(semi=True)
+    np.testing.assert_array_equal(prob.pipeline()[0], want)
+    # The rule picks a later position for some literals, so the grouping differs from the default.
+    assert any(oracle.literal_join_key(v, arg_types[r]) != oracle.literal_join_key(v) for r, v in lits)
+    got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, arg_types=arg_types, mode=mode, semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)

@@ -194,6 +194,32 @@ def test_label_aware_binding_table(vars_, rel_rows):
     assert prob.label_aware_binding_table(0, int(counts[2]) + 1, int(counts[3]))[0] == -1
 
 
+@pytest.mark.parametrize("vars_", [[0, 2], [1, 3], [2, 0], [0, 0], [1, 1, 0], [0, 3, 4], [-1, 1], [1, -1, 2]])
+@pytest.mark.parametrize("rel_rows", [30, 400])
+def test_create_binding_table_equals_bruteforce(vars_, rel_rows):
+    """CreateBindingTable (operations/MultiColumnHashJoin.cpp:26-115): variables below the binding
+    table's column count are join keys, the others new columns in position order.  The output is
+    the brute-force extended join as a multiset, on both probe-side branches (the relation is
+    smaller than the 150 binding rows at 30 rows and larger at 400)."""
+    arity = len(vars_)
+    binding, npos, rels, _ = random_problem(17 + arity, n=150, num_cols=2, rel_shapes=((arity, rel_rows),), domain=9)
+    prob = oracle.Problem(binding, npos, rels, [(0, vars_)])
+    cols = prob.binding_table(0)
+    C = len(binding)
+    keyed = [(i, v) for i, v in enumerate(vars_) if 0 <= v < C]
+    new = [i for i, v in enumerate(vars_) if not 0 <= v < C]
+    rel = rels[0]
+    want = []
+    for r in range(150):
+        for s in range(rel_rows):
+            if all(rel[i][s] == binding[v][r] for i, v in keyed):
+                want.append(tuple(int(binding[c][r]) for c in range(C)) + tuple(int(rel[i][s]) for i in new))
+    assert len(cols) == C + len(new)
+    got = list(zip(*[c.tolist() for c in cols])) if len(cols[0]) else []
+    assert len(want) > 0
+    assert sorted(got) == sorted(want)
+
+
 def test_semi_joins_and_dedup():
     rng = np.random.Generator(np.random.PCG64(9))
     pk = [rng.integers(0, 15, 200).astype(np.int32), rng.integers(0, 3, 200).astype(np.int32)]
@@ -214,3 +240,19 @@ def test_semi_joins_and_dedup():
         assert first[i] == (1 if is_set and key not in seen else 0)
         if is_set:
             seen.add(key)
+
+
+def test_pipeline_over_cached_partitions():
+    """Background partitions made once and reused by later calls (with different binding rows
+    over the same relations) give the counts of the pipeline that partitions inside the call."""
+    binding, npos, rels, lits = random_problem(13, n=600, rel_shapes=((2, 500), (3, 300)), domain=25)
+    prob = oracle.Problem(binding, npos, rels, lits)
+    cache = oracle.RelationCache(prob)
+    for rows in (600, 200):
+        part = oracle.Problem([c[:rows] for c in binding], min(npos, rows), rels, lits)
+        want, want_bits = part.bruteforce(semi=True)
+        got, bits, st = part.pipeline_cached(cache, semi=True)
+        assert st == 0
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(bits, want_bits)
+    cache.close()
