@@ -88,32 +88,68 @@ def measured_traffic(workload):
     return rec.get("traffic_bytes_per_launch") if rec else None
 
 
-def cpu_baseline(workload, fraction, log_fn, grouped=False):
-    """The oracle's restatement of the reference pipeline (1 thread) on the first `fraction`
-    of the binding rows with every relation and literal; `grouped`: the oracle's grouped-count
-    closed form instead, for inputs whose matching pairs the materialising pipeline cannot hold."""
+def cpu_baseline(workload, fractions, reps, log_fn, grouped=False):
+    """The oracle's restatement of the reference pipeline on one pinned host core, warm: the
+    relations' partitions are made before the timer starts, as the reference keeps them on the
+    fact tables across refinement steps (learner/CandidateLiteralEvaluator.cpp:291-299).  Each
+    of `fractions` (of the binding rows, every relation and literal) is timed `reps` times and
+    the median kept.  With two fractions the full step is extrapolated as fixed + slope * rows
+    (the relation rows' probes do not depend on the binding rows, the matches grow with them);
+    with one fraction of 1.0 the median is the full step.  `grouped`: the oracle's grouped-count
+    closed form instead, for inputs whose matching pairs the materialising pipeline cannot hold.
+    Returns (seconds per full step, description)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     nb = int(workload["binding"][0].numel())
-    rows = max(64, int(nb * fraction))
-    binding = [c[:rows].cpu().numpy() for c in workload["binding"]]
-    npos = min(rows, int(rows * workload["num_positive"] / nb))
     rels = [[c.cpu().numpy() for c in r] for r in workload["relations"]]
-    prob = oracle.Problem(binding, npos, rels, workload["literals"])
+    binding_all = [c.cpu().numpy() for c in workload["binding"]]
+    probs, rows_list = [], []
+    for f in fractions:
+        rows = min(nb, max(64, int(nb * f)))
+        npos = min(rows, int(rows * workload["num_positive"] / nb))
+        probs.append(oracle.Problem([c[:rows] for c in binding_all], npos, rels, workload["literals"]))
+        rows_list.append(rows)
     # One pinned host core, as the reference runs single-threaded.
     saved = os.sched_getaffinity(0)
     os.sched_setaffinity(0, {min(saved)})
     try:
         t0 = time.perf_counter()
-        if grouped:
-            prob.grouped()
-        else:
-            prob.pipeline(radix_bits=5, chunk_size=32768)
-        dt = time.perf_counter() - t0
+        cache = None if grouped else oracle.RelationCache(probs[0])
+        part_s = time.perf_counter() - t0
+        medians = []
+        for prob, rows in zip(probs, rows_list):
+            times = []
+            for k in range(reps):
+                t0 = time.perf_counter()
+                if grouped:
+                    prob.grouped()
+                else:
+                    prob.pipeline_cached(cache)
+                times.append(time.perf_counter() - t0)
+                log_fn("cpu baseline: %d binding rows, run %d/%d: %.2f s" % (rows, k + 1, reps, times[-1]))
+            medians.append(float(np.median(times)))
     finally:
         os.sched_setaffinity(0, saved)
-    log_fn("cpu baseline: %d literals over %d binding rows in %.2f s" % (len(workload["literals"]), rows, dt))
-    return dt, rows
+        if cache is not None:
+            cache.close()
+    method = ("oracle grouped-count closed form (the reference's materialising pipeline cannot hold this input's "
+              "5.6e11 matching pairs)" if grouped else "oracle pipeline (reference algorithm)")
+    if len(fractions) == 1:
+        full = medians[0] * nb / rows_list[0]
+        desc = ("%s, 1 thread pinned to one core, relation partitions made before timing (%.2f s, not counted); "
+                "median of %d runs on %d of %d binding rows with all relations and literals: %.2f s"
+                % (method, part_s, reps, rows_list[0], nb, medians[0]))
+        if rows_list[0] != nb:
+            desc += "; scaled by %d/%d rows" % (nb, rows_list[0])
+        return full, desc
+    slope = (medians[1] - medians[0]) / (rows_list[1] - rows_list[0])
+    fixed = medians[0] - slope * rows_list[0]
+    full = fixed + slope * nb
+    desc = ("%s, 1 thread pinned to one core, relation partitions made before timing (%.2f s, not counted); "
+            "medians of %d runs with all relations and literals on %d and %d of %d binding rows: %.2f s and %.2f s; "
+            "full step extrapolated as fixed + slope * rows = %.2f s + %.3e s/row * %d = %.1f s"
+            % (method, part_s, reps, rows_list[0], rows_list[1], nb, medians[0], medians[1], fixed, slope, nb, full))
+    return full, desc
 
 
 def host_cpu():
@@ -156,9 +192,10 @@ def main():
                     default="step")
     ap.add_argument("--dist", choices=["uniform", "zipf"], default="zipf")
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-fraction", type=float, default=None,
-                    help="binding rows the CPU baseline runs on (default: about 10 s of CPU work: 1/256 of the "
-                         "refinement step, all of the microbench)")
+    ap.add_argument("--cpu-fractions", type=float, nargs="+", default=None,
+                    help="binding-row fractions the CPU baseline is timed at (default: 1/256 and 1/64 of the "
+                         "refinement step, extrapolated to the full step; all of the microbench)")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="runs per CPU fraction (the median is kept)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
@@ -292,16 +329,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        frac = args.cpu_fraction if args.cpu_fraction is not None else (1.0 / 256 if args.workload == "step" else 1.0)
+        fracs = args.cpu_fractions or ([1.0 / 256, 1.0 / 64] if args.workload == "step" else [1.0])
         grouped = args.workload == "micro" and args.dist == "zipf"
-        dt, rows = cpu_baseline(w, frac, log, grouped=grouped)
-        nb = int(w["binding"][0].numel())
-        cpu = {"value": L / (dt * nb / rows), "unit": "literals/s", "cores": 1, "kind": "port", **host_cpu(),
-               "sample": "%s (1 thread pinned to one core) on the first %d of %d binding rows with all relations and "
-                         "literals: %.2f s; value scales that time by %d/%d rows"
-                         % ("oracle grouped-count closed form (the reference's materialising pipeline cannot hold "
-                            "this input's 5.6e11 matching pairs)" if grouped else
-                            "oracle pipeline (reference algorithm)", rows, nb, dt, nb, rows)}
+        full_s, desc = cpu_baseline(w, fracs, max(1, args.cpu_reps), log, grouped=grouped)
+        cpu = {"value": L / full_s, "unit": "literals/s", "cores": 1, "kind": "port", **host_cpu(),
+               "seconds_per_step": full_s, "sample": desc}
 
     out = {
         "metric": metric,

@@ -119,12 +119,13 @@ typedef struct {
   int mode;              /* qf_eval_mode */
   int radix_bits;        /* 0 = from partition_rows; else partition bits (at most 16) for both
                             sides */
-  int tile_rows;         /* 0 = autotuned; binding rows per work item */
+  int tile_rows;         /* binding rows per work item; 0 = 1024, or the autotuner's choice when
+                            autotune = 1 */
   int use_graph;         /* batched mode: capture the step's device work as a hipGraph and
                             replay it when the same call repeats (same columns, literals,
                             options); needs reuse_partitions = 1 */
   int reuse_partitions;  /* 1: cache relation partitions and their tables in the context, keyed
-                            by (column pointer, rows, join column, radix bits), as the reference
+                            by (column addresses, rows, join column, radix bits), as the reference
                             caches them on the fact table (learner/CandidateLiteralEvaluator.cpp:
                             291-299).  Only for relations that stay unchanged at that address
                             until qf_context_trim; default 0. */

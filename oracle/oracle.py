@@ -40,6 +40,12 @@ def lib():
         _lib.orc_create_binding_table.restype = ctypes.c_int64
         _lib.orc_generate_plan.restype = ctypes.c_int
         _lib.orc_literal_join_key.restype = ctypes.c_int
+        _lib.orc_relation_cache_create.restype = ctypes.c_void_p
+        _lib.orc_relation_cache_create.argtypes = []
+        _lib.orc_relation_cache_destroy.restype = None
+        _lib.orc_relation_cache_destroy.argtypes = [ctypes.c_void_p]
+        _lib.orc_relation_cache_fill.restype = None
+        _lib.orc_evaluate_pipeline_cached.restype = ctypes.c_int
     return _lib
 
 
@@ -151,6 +157,13 @@ class Problem:
                                          _p(bits) if semi else None)
         return counts[: self.L], (bits[: self.L] if semi else None), st
 
+    def pipeline_cached(self, cache, radix_bits=5, chunk_size=32768, semi=False):
+        """The pipeline over a RelationCache's background partitions (made outside this call)."""
+        counts, bits = self._out(semi)
+        st = lib().orc_evaluate_pipeline_cached(ctypes.byref(self.inp), ctypes.c_void_p(cache.h), radix_bits,
+                                                chunk_size, _p(counts), _p(bits) if semi else None)
+        return counts[: self.L], (bits[: self.L] if semi else None), st
+
     def bruteforce(self, semi=False):
         counts, bits = self._out(semi)
         lib().orc_evaluate_bruteforce(ctypes.byref(self.inp), _p(counts), _p(bits) if semi else None)
@@ -244,3 +257,22 @@ def dedup_after_semi_join(key_columns, bits):
     out = np.zeros(max(1, n), dtype=np.uint8)
     lib().orc_dedup_after_semi_join(k.ptrs, len(k.arrs), ctypes.c_int64(n), _p(b), _p(out))
     return out[:n]
+
+
+class RelationCache:
+    """Background partitions of every (relation, join key) a problem's literals use, made once
+    (learner/CandidateLiteralEvaluator.cpp:291-299 keeps them on the fact tables); valid for
+    problems over the same relation arrays."""
+
+    def __init__(self, problem, radix_bits=5):
+        self.h = lib().orc_relation_cache_create()
+        self.radix_bits = radix_bits
+        lib().orc_relation_cache_fill(ctypes.c_void_p(self.h), ctypes.byref(problem.inp), radix_bits)
+
+    def close(self):
+        if self.h:
+            lib().orc_relation_cache_destroy(ctypes.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -515,8 +515,44 @@ int orc_generate_plan(int arity, int join_key, const int* lit_vars, int n, int m
   return (int)p.left.size();
 }
 
+typedef std::map<std::pair<int, int>, Partitions> RelationPartitions;  // TableView::partitions_at cache
+
+struct orc_relation_cache {
+  RelationPartitions parts;
+};
+
+static int evaluate_pipeline(const orc_eval_input* in, int radix_bits, int chunk_size, RelationPartitions& bg_cache,
+                             uint64_t* counts, uint64_t* semi_bits);
+
 int orc_evaluate_pipeline(const orc_eval_input* in, int radix_bits, int chunk_size,
                           uint64_t* counts, uint64_t* semi_bits) {
+  RelationPartitions bg_cache;
+  return evaluate_pipeline(in, radix_bits, chunk_size, bg_cache, counts, semi_bits);
+}
+
+orc_relation_cache* orc_relation_cache_create(void) { return new orc_relation_cache(); }
+void orc_relation_cache_destroy(orc_relation_cache* c) { delete c; }
+
+// The background partitions of every (relation, join key) the literals use, made before any
+// evaluation, as the evaluator keeps them on the fact tables across refinement steps
+// (learner/CandidateLiteralEvaluator.cpp:291-299).
+void orc_relation_cache_fill(orc_relation_cache* c, const orc_eval_input* in, int radix_bits) {
+  for (int l = 0; l < in->num_literals; ++l) {
+    const int rel = in->lit_rel[l];
+    const int jk = literal_join_key_impl(in->rel_arity[rel], lit_vars_of(in, l), rel_types(in, rel));
+    if (jk < 0) continue;
+    const auto key = std::make_pair(rel, jk);
+    if (!c->parts.count(key)) c->parts.emplace(key, partition_column(rel_col(in, rel, jk), in->rel_rows[rel], radix_bits));
+  }
+}
+
+int orc_evaluate_pipeline_cached(const orc_eval_input* in, orc_relation_cache* c, int radix_bits, int chunk_size,
+                                 uint64_t* counts, uint64_t* semi_bits) {
+  return evaluate_pipeline(in, radix_bits, chunk_size, c->parts, counts, semi_bits);
+}
+
+static int evaluate_pipeline(const orc_eval_input* in, int radix_bits, int chunk_size, RelationPartitions& bg_cache,
+                             uint64_t* counts, uint64_t* semi_bits) {
   const int L = in->num_literals;
   const int64_t N = in->num_bindings;
   std::vector<LiteralCounts> out(L);
@@ -535,7 +571,6 @@ int orc_evaluate_pipeline(const orc_eval_input* in, int radix_bits, int chunk_si
     if (jk[l] < 0) continue;  // all-unbound literals are never enumerated
     by_var[lit_vars_of(in, l)[jk[l]]].push_back(l);
   }
-  std::map<std::pair<int, int>, Partitions> bg_cache;  // TableView::partitions_at cache
   std::vector<int32_t*> bcols(in->num_binding_columns);
   for (int c = 0; c < in->num_binding_columns; ++c) bcols[c] = const_cast<int32_t*>(in->binding_columns[c]);
 

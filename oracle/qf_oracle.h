@@ -77,6 +77,16 @@ typedef struct {
  * Returns 0, or -1 when a count does not fit the reference's int32 size_type. */
 int orc_evaluate_pipeline(const orc_eval_input* in, int radix_bits, int chunk_size,
                           uint64_t* counts, uint64_t* semi_bits);
+/* The same pipeline over background partitions made once (RadixPartition of every (relation,
+ * join key) the literals use) and kept across calls, as CandidateLiteralEvaluator keeps them on
+ * the fact tables (learner/CandidateLiteralEvaluator.cpp:291-299).  A cache belongs to one set of
+ * relations (the same rel_columns and rel_rows in every call). */
+typedef struct orc_relation_cache orc_relation_cache;
+orc_relation_cache* orc_relation_cache_create(void);
+void orc_relation_cache_destroy(orc_relation_cache* cache);
+void orc_relation_cache_fill(orc_relation_cache* cache, const orc_eval_input* in, int radix_bits);
+int orc_evaluate_pipeline_cached(const orc_eval_input* in, orc_relation_cache* cache, int radix_bits,
+                                 int chunk_size, uint64_t* counts, uint64_t* semi_bits);
 /* Nested-loop definition of the four counts (brute force; tiny inputs only). */
 void orc_evaluate_bruteforce(const orc_eval_input* in, uint64_t* counts, uint64_t* semi_bits);
 /* Grouped-count closed form: cnt(r) = #{s : P[B](s) = T[v_B](r)}; any skew. */

@@ -1087,7 +1087,8 @@ struct RelPart {
 };
 
 struct RelCache {
-  std::map<std::tuple<const void*, int64_t, int, int>, std::shared_ptr<RelPart>> parts;
+  // Key: rows, join column, radix bits, then every column address.
+  std::map<std::vector<int64_t>, std::shared_ptr<RelPart>> parts;
 };
 
 static RelCache& rel_cache(Context* ctx) {
@@ -1097,7 +1098,8 @@ static RelCache& rel_cache(Context* ctx) {
 
 static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table& rel, int join_col, int R,
                                                    bool cache) {
-  const auto key = std::make_tuple((const void*)rel.columns[0], rel.num_rows, join_col, R);
+  std::vector<int64_t> key{rel.num_rows, join_col, R};
+  for (int c = 0; c < rel.num_columns; ++c) key.push_back((int64_t)(uintptr_t)rel.columns[c]);
   RelCache& rc = rel_cache(ctx);
   if (cache) {
     auto it = rc.parts.find(key);
@@ -1711,8 +1713,10 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
     for (int c = 0; c < binding.num_columns; ++c) key.push_back((int64_t)(uintptr_t)binding.columns[c]);
     key.push_back((int64_t)specs.size());
     for (const GroupSpec& g : specs) {
-      key.insert(key.end(), {(int64_t)(uintptr_t)relations[g.relation].columns[0], g.join_col, g.join_var,
+      const qf_table& rel = relations[g.relation];
+      key.insert(key.end(), {(int64_t)rel.num_rows, (int64_t)rel.num_columns, g.join_col, g.join_var,
                              (int64_t)g.lits.size()});
+      for (int c = 0; c < rel.num_columns; ++c) key.push_back((int64_t)(uintptr_t)rel.columns[c]);
       for (const LitSpec& l : g.lits) {
         key.insert(key.end(), {l.slot, (int64_t)l.pcols.size(), (int64_t)l.eqs.size()});
         for (size_t k = 0; k < l.pcols.size(); ++k) key.insert(key.end(), {l.pcols[k], l.bcols[k]});

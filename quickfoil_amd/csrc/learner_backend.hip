@@ -48,15 +48,29 @@ qf_table view(const DevTable* t, std::vector<const int32_t*>* ptrs, int ncols) {
   return q;
 }
 
+int be_release(void* self, void* table) {
+  DeviceBackend* b = static_cast<DeviceBackend*>(self);
+  DevTable* t = static_cast<DevTable*>(table);
+  if (!t) return QF_OK;
+  (void)qf_synchronize(b->ctx);
+  for (int32_t* p : t->cols) (void)hipFree(p);
+  delete t;
+  return QF_OK;
+}
+
 int be_upload(void* self, const int32_t* const* cols, int ncols, int64_t rows, void** out) {
   DeviceBackend* b = static_cast<DeviceBackend*>(self);
   DevTable* t = nullptr;
   int st = alloc_table(ncols, rows, &t);
   if (st != QF_OK) return st;
-  if (qf_synchronize(b->ctx) != QF_OK) return QF_ERR_DEVICE;
-  for (int c = 0; c < ncols; ++c)
+  if (qf_synchronize(b->ctx) != QF_OK) st = QF_ERR_DEVICE;
+  for (int c = 0; c < ncols && st == QF_OK; ++c)
     if (rows > 0 && hipMemcpy(t->cols[c], cols[c], (size_t)rows * 4, hipMemcpyHostToDevice) != hipSuccess)
-      return QF_ERR_DEVICE;
+      st = QF_ERR_DEVICE;
+  if (st != QF_OK) {
+    be_release(self, t);
+    return st;
+  }
   *out = t;
   return QF_OK;
 }
@@ -71,28 +85,22 @@ int be_download(void* self, void* table, int ncols, int64_t rows, int32_t* const
   return QF_OK;
 }
 
-int be_release(void* self, void* table) {
-  DeviceBackend* b = static_cast<DeviceBackend*>(self);
-  DevTable* t = static_cast<DevTable*>(table);
-  if (!t) return QF_OK;
-  (void)qf_synchronize(b->ctx);
-  for (int32_t* p : t->cols) (void)hipFree(p);
-  delete t;
-  return QF_OK;
-}
-
 int be_concat(void* self, void* a, int64_t ra, void* bb, int64_t rb, int ncols, void** out) {
   DeviceBackend* b = static_cast<DeviceBackend*>(self);
   DevTable* t = nullptr;
   int st = alloc_table(ncols, ra + rb, &t);
   if (st != QF_OK) return st;
-  if (qf_synchronize(b->ctx) != QF_OK) return QF_ERR_DEVICE;
-  for (int c = 0; c < ncols; ++c) {
+  if (qf_synchronize(b->ctx) != QF_OK) st = QF_ERR_DEVICE;
+  for (int c = 0; c < ncols && st == QF_OK; ++c) {
     if (ra > 0 && hipMemcpy(t->cols[c], static_cast<DevTable*>(a)->cols[c], (size_t)ra * 4, hipMemcpyDeviceToDevice))
-      return QF_ERR_DEVICE;
-    if (rb > 0 &&
-        hipMemcpy(t->cols[c] + ra, static_cast<DevTable*>(bb)->cols[c], (size_t)rb * 4, hipMemcpyDeviceToDevice))
-      return QF_ERR_DEVICE;
+      st = QF_ERR_DEVICE;
+    else if (rb > 0 &&
+             hipMemcpy(t->cols[c] + ra, static_cast<DevTable*>(bb)->cols[c], (size_t)rb * 4, hipMemcpyDeviceToDevice))
+      st = QF_ERR_DEVICE;
+  }
+  if (st != QF_OK) {
+    be_release(self, t);
+    return st;
   }
   *out = t;
   return QF_OK;
@@ -105,10 +113,16 @@ int be_evaluate(void* self, void* binding, int ncols, int64_t rows, int64_t npos
   DevTable* t = static_cast<DevTable*>(binding);
   qf_table bt = view(t, &p, ncols);
   bt.num_rows = rows;
+  // The relations never change while the backend lives, so their partitions and tables are
+  // cached in the context (the reference caches them on the fact table,
+  // learner/CandidateLiteralEvaluator.cpp:291-299); the destroy call drops them.
   qf_eval_options o;
   qf_eval_options_default(&o);
-  return qf_evaluate_literals(b->ctx, &bt, npos, b->rel_tables.data(), b->types.data(), (int)b->rel_tables.size(),
-                              lits, n, &o, counts, nullptr);
+  o.reuse_partitions = 1;
+  const int st = qf_evaluate_literals(b->ctx, &bt, npos, b->rel_tables.data(), b->types.data(),
+                                      (int)b->rel_tables.size(), lits, n, &o, counts, nullptr);
+  // A binding count above int32 is still exact in the uint64 counts the learner reads.
+  return st == QF_ERR_COUNT_RANGE ? QF_OK : st;
 }
 
 int be_extend(void* self, void* binding, int ncols, int64_t rows, int64_t npos, const qf_literal* lit, int64_t bpos,
@@ -194,18 +208,18 @@ extern "C" int qf_learner_device_backend_create(qf_context* ctx, int num_relatio
   b->ctx = ctx;
   b->relations.resize(num_relations);
   b->rel_ptrs.resize(num_relations);
+  auto fail = [&](int st) {
+    for (DevTable& t : b->relations)
+      for (int32_t* p : t.cols) (void)hipFree(p);
+    delete b;
+    return st;
+  };
   for (int r = 0; r < num_relations; ++r) {
-    if (arities[r] < 1 || arities[r] > QF_MAX_ARITY) {
-      delete b;
-      return QF_ERR_INVALID;
-    }
+    if (arities[r] < 1 || arities[r] > QF_MAX_ARITY) return fail(QF_ERR_INVALID);
     DevTable* t = nullptr;
     void* h = nullptr;
     int st = be_upload(b, relation_columns[r], arities[r], relation_rows[r], &h);
-    if (st != QF_OK) {
-      delete b;
-      return st;
-    }
+    if (st != QF_OK) return fail(st);
     t = static_cast<DevTable*>(h);
     b->relations[r] = *t;
     delete t;
@@ -230,7 +244,8 @@ extern "C" int qf_learner_device_backend_create(qf_context* ctx, int num_relatio
 extern "C" void qf_learner_device_backend_destroy(qf_learner_backend* backend) {
   if (!backend || !backend->self) return;
   DeviceBackend* b = static_cast<DeviceBackend*>(backend->self);
-  (void)qf_synchronize(b->ctx);
+  // Cached partitions are keyed by column address; a later allocation may reuse these.
+  (void)qf_context_trim(b->ctx);
   for (DevTable& t : b->relations)
     for (int32_t* p : t.cols) (void)hipFree(p);
   delete b;
