@@ -75,6 +75,17 @@ typedef struct {
 int qf_radix_partition(qf_context* ctx, const int32_t* d_values, int64_t n, int radix_bits,
                        int32_t* d_out_pairs, int64_t* d_offsets);
 
+/* qf_radix_partition for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops or harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same arguments and results as qf_radix_partition for n <= QF_NOALLOC_MAX_ROWS; larger inputs
+ * return QF_ERR_INVALID.  It calls no device allocator: its temporaries live in a fixed 36 MiB
+ * device array that the library holds on every GPU it runs on until the process ends.  All calls
+ * on one GPU share that array, so calls that overlap on one GPU (other contexts or streams)
+ * race; calls ordered on one stream do not. */
+#define QF_NOALLOC_MAX_ROWS 4194304
+int qf_radix_partition_noalloc(qf_context* ctx, const int32_t* d_values, int64_t n, int radix_bits,
+                               int32_t* d_out_pairs, int64_t* d_offsets);
+
 /* Bucket layout of BuildHashTableOnPartitions: partition p gets P2(n_p) buckets (none when
  * empty; storage/FoilHashTable.cpp:34-49).  Host-only: h_offsets are the partition offsets,
  * h_bucket_offsets receives num_partitions + 1 entries.  Returns the total bucket count

@@ -107,6 +107,7 @@ SIGNATURES = {
     "qf_context_reset_timing": (_I, [_P]),
     "qf_context_phase_times": (_I, [_P, _P, _P, _P, _I]),
     "qf_radix_partition": (_I, [_P, _P, _I64, _I, _P, _P]),
+    "qf_radix_partition_noalloc": (_I, [_P, _P, _I64, _I, _P, _P]),
     "qf_partition_bucket_layout": (_I64, [_P, _I, _P]),
     "qf_build_hash_table_on_partitions": (_I, [_P, _P, _P, _I, _P, _I64, _I64, _P, _P]),
     "qf_build_hash_table_on_table": (_I, [_P, ctypes.POINTER(QfTable), _P, _P]),

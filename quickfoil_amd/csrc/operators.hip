@@ -13,6 +13,8 @@
 
 namespace qf {
 
+void noalloc_scratch_setup(int device);  // noalloc.hip
+
 static int bits_for(uint64_t n) {
   int b = 0;
   while (b < 32 && ((uint64_t)1 << b) < n) ++b;
@@ -192,6 +194,7 @@ int qf_context_create(int device, qf_context** out) {
   }
   ctx->own_stream = true;
   ctx->pool.reset(new Pool(device));
+  noalloc_scratch_setup(device);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
   *out = reinterpret_cast<qf_context*>(ctx);

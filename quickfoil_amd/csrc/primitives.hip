@@ -177,7 +177,16 @@ __global__ void k_scan_down(const int64_t* in, int64_t* out, int64_t n, const in
   }
 }
 
-void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n) {
+static size_t aligned256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t scan_scratch_bytes(int64_t n) {
+  size_t total = 0;
+  for (int64_t blocks = ceil_div(n, SCAN_TILE); blocks > 1; blocks = ceil_div(blocks, SCAN_TILE))
+    total += aligned256((size_t)blocks * sizeof(int64_t));
+  return total;
+}
+
+void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n, Scratch* scratch) {
   if (n <= 0) return;
   const int64_t blocks = ceil_div(n, SCAN_TILE);
   if (blocks == 1) {
@@ -185,13 +194,18 @@ void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n
     QF_CHECK_HIP(hipGetLastError());
     return;
   }
-  DevBuf sums(ctx, blocks * sizeof(int64_t));
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(SCAN_THREADS), 0, ctx->stream, in, n,
-                     sums.as<int64_t>());
+  DevBuf pooled;
+  int64_t* sums = nullptr;
+  if (scratch) {
+    sums = static_cast<int64_t*>(scratch->take(blocks * sizeof(int64_t)));
+  } else {
+    pooled = DevBuf(ctx, blocks * sizeof(int64_t));
+    sums = pooled.as<int64_t>();
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(SCAN_THREADS), 0, ctx->stream, in, n, sums);
   QF_CHECK_HIP(hipGetLastError());
-  scan_exclusive_i64(ctx, sums.as<int64_t>(), sums.as<int64_t>(), blocks);
-  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)blocks), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
-                     sums.as<int64_t>());
+  scan_exclusive_i64(ctx, sums, sums, blocks, scratch);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)blocks), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n, sums);
   QF_CHECK_HIP(hipGetLastError());
 }
 
@@ -471,43 +485,81 @@ static void launch_scatter(Context* ctx, const PassArgs& a, const int64_t* bases
   hipLaunchKernelGGL(k_part_scatter<NC>, dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
 }
 
-static void run_pass(Context* ctx, PassArgs a) {
+static void run_pass(Context* ctx, PassArgs a, Scratch* scratch) {
   a.num_tiles = std::max<int64_t>(1, ceil_div(a.n, PT_TILE));
   const int nb = 1 << a.bits;
-  DevBuf counts(ctx, (size_t)nb * a.num_tiles * sizeof(int64_t));
+  const size_t count_bytes = (size_t)nb * a.num_tiles * sizeof(int64_t);
+  DevBuf pooled;
+  int64_t* counts = nullptr;
+  if (scratch) {
+    counts = static_cast<int64_t*>(scratch->take(count_bytes));
+  } else {
+    pooled = DevBuf(ctx, count_bytes);
+    counts = pooled.as<int64_t>();
+  }
   hipLaunchKernelGGL(k_part_hist, dim3((unsigned)ceil_div(a.num_tiles, PT_WAVES)), dim3(PT_THREADS), 0,
-                     ctx->stream, a, counts.as<int64_t>());
+                     ctx->stream, a, counts);
   QF_CHECK_HIP(hipGetLastError());
-  scan_exclusive_i64(ctx, counts.as<int64_t>(), counts.as<int64_t>(), (int64_t)nb * a.num_tiles);
+  scan_exclusive_i64(ctx, counts, counts, (int64_t)nb * a.num_tiles, scratch);
   switch (a.ncols) {
-    case 1: launch_scatter<1>(ctx, a, counts.as<int64_t>()); break;
-    case 2: launch_scatter<2>(ctx, a, counts.as<int64_t>()); break;
-    case 3: launch_scatter<3>(ctx, a, counts.as<int64_t>()); break;
-    case 4: launch_scatter<4>(ctx, a, counts.as<int64_t>()); break;
-    case 5: launch_scatter<5>(ctx, a, counts.as<int64_t>()); break;
-    case 6: launch_scatter<6>(ctx, a, counts.as<int64_t>()); break;
-    case 7: launch_scatter<7>(ctx, a, counts.as<int64_t>()); break;
-    default: launch_scatter<8>(ctx, a, counts.as<int64_t>()); break;
+    case 1: launch_scatter<1>(ctx, a, counts); break;
+    case 2: launch_scatter<2>(ctx, a, counts); break;
+    case 3: launch_scatter<3>(ctx, a, counts); break;
+    case 4: launch_scatter<4>(ctx, a, counts); break;
+    case 5: launch_scatter<5>(ctx, a, counts); break;
+    case 6: launch_scatter<6>(ctx, a, counts); break;
+    case 7: launch_scatter<7>(ctx, a, counts); break;
+    default: launch_scatter<8>(ctx, a, counts); break;
   }
   QF_CHECK_HIP(hipGetLastError());
 }
 
+// Digit bits of each LSD pass: at most 8 per pass, split evenly (low part first).
+static std::vector<int> pass_bits(int total_bits) {
+  const int passes = std::max(1, (total_bits + 7) / 8);
+  std::vector<int> bits(passes, total_bits / passes);
+  for (int p = 0; p < total_bits % passes; ++p) ++bits[p];
+  return bits;
+}
+
+size_t partition_scratch_bytes(int64_t num_rows, int ncols, int total_bits) {
+  if (num_rows <= 0) return 0;
+  const std::vector<int> bits = pass_bits(total_bits);
+  const int passes = (int)bits.size();
+  size_t total = 0;
+  for (int s = 0; s < 2 && s < passes - 1; ++s) total += (size_t)ncols * aligned256((size_t)num_rows * sizeof(int32_t));
+  const int64_t tiles = std::max<int64_t>(1, ceil_div(num_rows, PT_TILE));
+  for (int b : bits) {
+    const int64_t m = ((int64_t)1 << b) * tiles;
+    total += aligned256((size_t)m * sizeof(int64_t)) + scan_scratch_bytes(m);
+  }
+  return total;
+}
+
 static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col,
                            bool gen_tid, int64_t num_rows, int mixed, int total_bits,
-                           int32_t* const* out_cols, int out_stride, int64_t* d_offsets) {
+                           int32_t* const* out_cols, int out_stride, int64_t* d_offsets, Scratch* scratch) {
   const int ncols = num_in_cols + (gen_tid ? 1 : 0);
   QF_REQUIRE(ncols <= PT_MAXCOLS, "too many columns to partition");
   QF_REQUIRE(total_bits >= 0 && total_bits <= 32, "bad radix bit count");
   QF_REQUIRE(key_col >= 0 && key_col < num_in_cols, "bad key column");
   if (num_rows > 0) {
-    const int passes = std::max(1, (total_bits + 7) / 8);
-    // Split the digit bits evenly over the passes (LSD order: low part first).
-    std::vector<int> bits(passes, total_bits / passes);
-    for (int p = 0; p < total_bits % passes; ++p) ++bits[p];
-    std::vector<DevBuf> tmp[2];
+    const std::vector<int> bits = pass_bits(total_bits);
+    const int passes = (int)bits.size();
+    // Intermediate columns of the LSD passes (two sets, used alternately).
+    std::vector<DevBuf> pooled;
+    std::vector<int32_t*> tmp[2];
     if (passes > 1)
       for (int s = 0; s < 2 && s < passes - 1; ++s)
-        for (int c = 0; c < ncols; ++c) tmp[s].emplace_back(ctx, (size_t)num_rows * sizeof(int32_t));
+        for (int c = 0; c < ncols; ++c) {
+          const size_t bytes = (size_t)num_rows * sizeof(int32_t);
+          if (scratch) {
+            tmp[s].push_back(static_cast<int32_t*>(scratch->take(bytes)));
+          } else {
+            pooled.emplace_back(ctx, bytes);
+            tmp[s].push_back(pooled.back().as<int32_t>());
+          }
+        }
     int lo = 0;
     for (int p = 0; p < passes; ++p) {
       PassArgs a{};
@@ -523,17 +575,17 @@ static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_
         a.tid_col = gen_tid ? num_in_cols : -1;
         if (gen_tid) a.in[num_in_cols] = nullptr;
       } else {
-        for (int c = 0; c < ncols; ++c) a.in[c] = tmp[(p - 1) & 1][c].as<int32_t>();
+        for (int c = 0; c < ncols; ++c) a.in[c] = tmp[(p - 1) & 1][c];
         a.tid_col = -1;
       }
       if (p == passes - 1) {
         for (int c = 0; c < ncols; ++c) a.out[c] = out_cols[c];
         a.out_stride = out_stride;
       } else {
-        for (int c = 0; c < ncols; ++c) a.out[c] = tmp[p & 1][c].as<int32_t>();
+        for (int c = 0; c < ncols; ++c) a.out[c] = tmp[p & 1][c];
         a.out_stride = 1;
       }
-      run_pass(ctx, a);
+      run_pass(ctx, a, scratch);
       lo += bits[p];
     }
   }
@@ -550,14 +602,14 @@ static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_
 
 void stable_partition(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col, bool gen_tid,
                       int64_t num_rows, int mixed, int total_bits, int32_t* const* out_cols, int out_stride,
-                      int64_t* d_offsets) {
+                      int64_t* d_offsets, Scratch* scratch) {
   partition_impl(ctx, in_cols, num_in_cols, key_col, gen_tid, num_rows, mixed, total_bits, out_cols, out_stride,
-                 d_offsets);
+                 d_offsets, scratch);
 }
 
 void stable_sort_by_key(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col, bool gen_tid,
                         int64_t num_rows, int key_bits, int32_t* const* out_cols) {
-  partition_impl(ctx, in_cols, num_in_cols, key_col, gen_tid, num_rows, 0, key_bits, out_cols, 1, nullptr);
+  partition_impl(ctx, in_cols, num_in_cols, key_col, gen_tid, num_rows, 0, key_bits, out_cols, 1, nullptr, nullptr);
 }
 
 }  // namespace qf

@@ -168,11 +168,28 @@ __host__ __device__ inline uint32_t next_pow2_u32(uint32_t v) {
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ------------------------------------------------------------------------------------------
+// A fixed device region that an operation carves its temporaries from, in order, instead of
+// taking them from the context pool (for entry points that allocate nothing).
+struct Scratch {
+  char* base = nullptr;
+  size_t capacity = 0;
+  size_t used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at + bytes > capacity) throw Error(QF_ERR_INVALID, "scratch region too small");
+    used = at + bytes;
+    return base + at;
+  }
+};
+
+// ------------------------------------------------------------------------------------------
 // Primitives implemented in primitives.hip.
 
 // Exclusive prefix sum of n int64 values (in and out may alias).  Returns nothing; the total
-// is out[n-1] + in[n-1] and is not computed here.
-void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n);
+// is out[n-1] + in[n-1] and is not computed here.  Temporaries come from `scratch` when given.
+void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n, Scratch* scratch = nullptr);
+// Bytes scan_exclusive_i64 takes from a Scratch for n values (256-byte aligned pieces).
+size_t scan_scratch_bytes(int64_t n);
 
 void fill_i32(Context* ctx, int32_t* p, int32_t v, int64_t n);
 void fill_i64(Context* ctx, int64_t* p, int64_t v, int64_t n);
@@ -193,7 +210,9 @@ struct DigitFn {
 // : k & ((1 << total_bits) - 1).
 void stable_partition(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col,
                       bool gen_tid, int64_t num_rows, int mixed, int total_bits,
-                      int32_t* const* out_cols, int out_stride, int64_t* d_offsets);
+                      int32_t* const* out_cols, int out_stride, int64_t* d_offsets, Scratch* scratch = nullptr);
+// Bytes stable_partition takes from a Scratch for num_rows rows of ncols written columns.
+size_t partition_scratch_bytes(int64_t num_rows, int ncols, int total_bits);
 
 // Stable sort of (key, payload...) rows by uint32 key values below 2^key_bits (raw digits).
 void stable_sort_by_key(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col,
