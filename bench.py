@@ -191,7 +191,11 @@ def main():
     ap.add_argument("--workload", choices=["step", "micro", "learn-kinship", "learn-bongard", "estimator"],
                     default="step")
     ap.add_argument("--dist", choices=["uniform", "zipf"], default="zipf")
-    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--fresh-steps", type=int, default=3,
+                    help="steps on a new binding table each (relations cached, host planning and eager launch)")
+    ap.add_argument("--extend-literals", type=int, default=4,
+                    help="literals whose binding-set extension (CreateLabelAwareBindingTables) is timed")
     ap.add_argument("--cpu-fractions", type=float, nargs="+", default=None,
                     help="binding-row fractions the CPU baseline is timed at (default: 1/256 and 1/64 of the "
                          "refinement step, extrapolated to the full step; all of the microbench)")
@@ -295,19 +299,68 @@ def main():
     assert np.array_equal(eager_counts, ref_counts), "eager and graph counts differ"
     phases = ctx.phase_times()
 
-    # Plain form: one launch per literal, same partitions.
+    # A learner's step: a new binding table (new device columns) against cached relations, so
+    # host planning, the binding partition and an eager launch run every time (no graph replay).
+    fresh_s = None
+    if args.fresh_steps > 0:
+        times = []
+        for i in range(args.fresh_steps):
+            fresh = [c.clone() for c in w["binding"]]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fresh_counts, _ = ctx.evaluate_literals(fresh, w["num_positive"], w["relations"], lits, mode="batched",
+                                                    reuse_partitions=True, radix_bits=tune["radix_bits"],
+                                                    tile_rows=tune["tile_rows"], partition_rows=tune["partition_rows"])
+            times.append(time.perf_counter() - t0)
+            assert np.array_equal(fresh_counts, ref_counts), "counts on a new binding table differ"
+            del fresh
+        fresh_s = float(np.median(times))
+        log("new-binding-table step: %.2f ms (median of %d)" % (fresh_s * 1e3, len(times)))
+
+    # Plain form: one launch per literal, same partitions; its kernels' own time from the phase
+    # events beside the wall time (which includes one host planning pass per literal group).
     plain_s = None
+    plain_kernel_ms = None
     if args.plain_steps > 0:
-        ctx.reset_timing()
         step("plain_literal")  # warm its code path
         torch.cuda.synchronize()
+        ctx.reset_timing()
         t_start = time.perf_counter()
         for i in range(args.plain_steps):
             plain_counts = step("plain_literal")
             log("plain step %d done" % (i + 1))
         torch.cuda.synchronize()
         plain_s = (time.perf_counter() - t_start) / args.plain_steps
+        plain_kernel_ms = sum(v[0] for v in ctx.phase_times().values()) / args.plain_steps
         assert np.array_equal(plain_counts, ref_counts), "plain and batched counts differ"
+
+    # Binding-set extension (CreateLabelAwareBindingTables) for a few literals whose new binding
+    # tables are large, into preallocated outputs.
+    extension = None
+    if args.extend_literals > 0 and args.workload == "step":
+        sizes = ref_counts[:, 2].astype(np.int64) + ref_counts[:, 3].astype(np.int64)
+        cand = [i for i in np.argsort(-sizes) if sizes[i] <= 40_000_000][: args.extend_literals]
+        rows_max = int(max(sizes[i] for i in cand)) if cand else 0
+        arena = [torch.empty(max(1, rows_max), dtype=torch.int32, device=dev) for _ in range(len(w["binding"]) + 3)]
+        ext_ms, ext_rows = 0.0, 0
+        for i in cand:
+            rel, vars_ = w["literals"][i]
+            args_ext = (w["binding"], w["num_positive"], w["relations"][rel], vars_, int(ref_counts[i, 2]),
+                        int(ref_counts[i, 3]))
+            ctx.create_label_aware_binding_table(*args_ext, out=arena)  # warm
+            torch.cuda.synchronize()
+            reps = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ctx.create_label_aware_binding_table(*args_ext, out=arena)
+                torch.cuda.synchronize()
+                reps.append(time.perf_counter() - t0)
+            ext_ms += float(np.median(reps)) * 1e3
+            ext_rows += int(sizes[i])
+        del arena
+        extension = {"literals": len(cand), "rows": ext_rows, "ms": ext_ms,
+                     "rows_per_s": ext_rows / (ext_ms * 1e-3) if ext_ms > 0 else None}
+        log("extension: %d literals, %d rows in %.2f ms" % (len(cand), ext_rows, ext_ms))
 
     if args.copy_calibration:
         # A plain device copy of known size, for scaling rocprof FETCH_SIZE / WRITE_SIZE.
@@ -353,7 +406,12 @@ def main():
                    "facts_per_partition": tune["partition_rows"] or 256, "tile_rows": tune["tile_rows"] or 1024},
         "plain_literals_per_s": (L / plain_s) if plain_s else None,
         "plain_ms_per_step": (plain_s * 1e3) if plain_s else None,
+        "plain_kernel_ms_per_step": plain_kernel_ms,
+        "plain_steps": args.plain_steps,
         "batched_over_plain": (plain_s / batched_s) if plain_s else None,
+        "fresh_binding_ms_per_step": (fresh_s * 1e3) if fresh_s else None,
+        "fresh_binding_literals_per_s": (L / fresh_s) if fresh_s else None,
+        "extension": extension,
         "cold_step_ms": cold_s * 1e3,
         "autotune_ms": (autotune_s * 1e3) if autotune_s is not None else None,
         "copy_ceiling_GBs": copy_gbs,

@@ -169,6 +169,17 @@ int qf_evaluate_literals(qf_context* ctx, const qf_table* binding, int64_t num_p
                          const qf_eval_options* opts, uint64_t* h_counts,
                          uint64_t* d_semi_bits);
 
+/* CandidateLiteralEvaluator::Evaluate over a clause whose positive and negative bindings are two
+ * tables (the restart clause, learner/QuickFoil.cpp:127-140, schema/FoilClause.hpp:152-161): the
+ * passes of learner/CandidateLiteralEvaluator.cpp:345-405 (ExecuteOnPositives, then
+ * ExecuteOnNegatives).  Counts as qf_evaluate_literals over the two tables concatenated
+ * (positives first), without the copy.  d_semi_bits_pos / d_semi_bits_neg (nullable): per
+ * literal, ceil(rows/64) words over the rows of each table.  opts->use_graph is ignored. */
+int qf_evaluate_literals_split(qf_context* ctx, const qf_table* positives, const qf_table* negatives,
+                               const qf_table* relations, const int* arg_types, int num_relations,
+                               const qf_literal* literals, int num_literals, const qf_eval_options* opts,
+                               uint64_t* h_counts, uint64_t* d_semi_bits_pos, uint64_t* d_semi_bits_neg);
+
 /* CountAggregator::Execute over an explicit PredicateEvaluationPlan for one join group
  * (operations/CountAggregator.cpp:112-237, learner/PredicateEvaluationPlan.hpp:31-112).
  * Atoms are equalities relation[atom_bg_col[k]] == binding[atom_binding_col[k]] beside the join

@@ -115,6 +115,9 @@ SIGNATURES = {
     "qf_autotune_choice": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double)]),
     "qf_evaluate_literals": (_I, [_P, ctypes.POINTER(QfTable), _I64, ctypes.POINTER(QfTable), _P, _I,
                                   ctypes.POINTER(QfLiteral), _I, ctypes.POINTER(QfEvalOptions), _P, _P]),
+    "qf_evaluate_literals_split": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), ctypes.POINTER(QfTable),
+                                        _P, _I, ctypes.POINTER(QfLiteral), _I, ctypes.POINTER(QfEvalOptions), _P, _P,
+                                        _P]),
     "qf_count_aggregate": (_I, [_P, ctypes.POINTER(QfTable), _I64, ctypes.POINTER(QfTable), _I,
                                 ctypes.POINTER(QfJoinGroupPlan), _I, _I, ctypes.POINTER(QfEvalOptions), _P, _P]),
     "qf_create_label_aware_binding_table": (_I, [_P, ctypes.POINTER(QfTable), _I64, ctypes.POINTER(QfTable),
@@ -327,6 +330,40 @@ class Context:
             bits = bits[: L * ((n + 63) // 64)].view(L, (n + 63) // 64)
         return counts[:L], bits
 
+    def evaluate_literals_split(self, positive_columns, negative_columns, relations, literals, arg_types=None,
+                                mode="batched", radix_bits=0, tile_rows=0, reuse_partitions=False, semi_bits=False,
+                                check_range=False, partition_rows=0):
+        """Counts over a clause whose positive and negative bindings are separate tables (the two
+        passes); returns (counts[L,4], positive bits or None, negative bits or None)."""
+        t = self.torch
+        bp, bn = _TableHolder(positive_columns), _TableHolder(negative_columns)
+        rels = [_TableHolder(r) for r in relations]
+        rel_arr = (QfTable * max(1, len(rels)))(*[r.t for r in rels])
+        if not isinstance(literals, LiteralArray):
+            literals = LiteralArray(literals)
+        L = len(literals)
+        lit_arr = literals.pointer() if L > 0 else (QfLiteral * 1)()
+        types = None
+        types_ptr = ctypes.c_void_p(0)
+        if arg_types is not None:
+            types = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.int32) for x in arg_types]))
+            types_ptr = types.ctypes.data_as(ctypes.c_void_p)
+        counts = np.zeros((max(1, L), 4), dtype=np.uint64)
+        bits = [None, None]
+        words = [(bp.t.num_rows + 63) // 64, (bn.t.num_rows + 63) // 64]
+        if semi_bits:
+            dev = positive_columns[0].device
+            bits = [t.empty(max(1, L * w), dtype=t.int64, device=dev) for w in words]
+        o = self._options(mode, radix_bits, tile_rows, False, reuse_partitions, partition_rows)
+        st = self.lib.qf_evaluate_literals_split(self.h, ctypes.byref(bp.t), ctypes.byref(bn.t), rel_arr, types_ptr,
+                                                 len(rels), lit_arr, L, ctypes.byref(o),
+                                                 counts.ctypes.data_as(ctypes.c_void_p), _ptr(bits[0]), _ptr(bits[1]))
+        if st != QF_OK and not (st == 5 and not check_range):
+            self._check(st)
+        if semi_bits:
+            bits = [b[: L * w].view(L, w) for b, w in zip(bits, words)]
+        return counts[:L], bits[0], bits[1]
+
     def count_aggregate(self, binding_columns, num_positive, relations, plans, num_slots, mode="batched",
                         radix_bits=0, tile_rows=0, semi_bits=False):
         """plans: list of dicts with relation, join_column, binding_column, atoms [(bg, bind)],
@@ -379,7 +416,9 @@ class Context:
         return lit
 
     def create_label_aware_binding_table(self, binding_columns, num_positive, relation_columns, vars_,
-                                         num_binding_pos, num_binding_neg):
+                                         num_binding_pos, num_binding_neg, out=None):
+        """The extended binding table (binding columns, then one per new variable), positives then
+        negatives; `out`: int32 device buffers to write into (one per output column, long enough)."""
         t = self.torch
         b = _TableHolder(binding_columns)
         r = _TableHolder(relation_columns)
@@ -387,7 +426,12 @@ class Context:
         n_new = sum(1 for v in vars_ if v < 0)
         rows = num_binding_pos + num_binding_neg
         dev = binding_columns[0].device
-        outs = [t.empty(max(1, rows), dtype=t.int32, device=dev) for _ in range(len(binding_columns) + n_new)]
+        ncols = len(binding_columns) + n_new
+        if out is not None:
+            assert len(out) >= ncols and all(o.numel() >= rows for o in out[:ncols]), "output buffers too small"
+            outs = list(out[:ncols])
+        else:
+            outs = [t.empty(max(1, rows), dtype=t.int32, device=dev) for _ in range(ncols)]
         arr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
         self._check(self.lib.qf_create_label_aware_binding_table(self.h, ctypes.byref(b.t), num_positive,
                                                                  ctypes.byref(r.t), ctypes.byref(lit),

@@ -1823,6 +1823,69 @@ static void check_table(const qf_table* t, const char* what) {
     QF_REQUIRE(t->num_rows == 0 || t->columns[c] != nullptr, std::string(what) + " has a null column");
 }
 
+}  // extern "C"
+
+namespace qf {
+
+// Literal groups of a step: literals grouped by (relation, join position, clause join variable)
+// (learner/QuickFoil.cpp:142-161, CandidateLiteralEvaluator.cpp:281-289); PLAIN_LITERAL keeps one
+// group per literal.
+static std::vector<GroupSpec> literal_groups(int binding_columns, const qf_table* relations, const int* arg_types,
+                                             int num_relations, const qf_literal* literals, int num_literals,
+                                             const qf_eval_options& o) {
+  std::vector<int> type_start(num_relations + 1, 0);
+  for (int r = 0; r < num_relations; ++r) {
+    check_table(&relations[r], "relation");
+    QF_REQUIRE(relations[r].num_columns <= QF_MAX_ARITY, "relation arity above QF_MAX_ARITY");
+    type_start[r + 1] = type_start[r] + relations[r].num_columns;
+  }
+  std::vector<GroupSpec> specs;
+  std::map<std::tuple<int, int, int>, int> index;
+  for (int l = 0; l < num_literals; ++l) {
+    const qf_literal& lit = literals[l];
+    QF_REQUIRE(lit.relation >= 0 && lit.relation < num_relations, "literal relation out of range");
+    const int arity = relations[lit.relation].num_columns;
+    const int* types = arg_types ? arg_types + type_start[lit.relation] : nullptr;
+    for (int i = 0; i < arity; ++i)
+      QF_REQUIRE(lit.vars[i] >= -1 && lit.vars[i] < binding_columns, "literal variable out of range");
+    const int jk = literal_join_key(lit, arity, types);
+    QF_REQUIRE(jk >= 0, "literal has no bound variable");
+    LitSpec ls;
+    ls.slot = l;
+    for (int i = 0; i < arity; ++i)
+      if (lit.vars[i] >= 0) {
+        ls.pcols.push_back(i);
+        ls.bcols.push_back(lit.vars[i]);
+      }
+    normalise_key(ls);
+    const auto key = std::make_tuple(lit.relation, jk, lit.vars[jk]);
+    int gi;
+    if (o.mode == QF_EVAL_PLAIN_LITERAL || !index.count(key)) {
+      gi = (int)specs.size();
+      GroupSpec g;
+      g.relation = lit.relation;
+      g.join_col = jk;
+      g.join_var = lit.vars[jk];
+      specs.push_back(g);
+      if (o.mode != QF_EVAL_PLAIN_LITERAL) index[key] = gi;
+    } else {
+      gi = index[key];
+    }
+    specs[gi].lits.push_back(ls);
+  }
+  return specs;
+}
+
+static void check_count_range(const uint64_t* h_counts, int num_literals) {
+  for (int l = 0; l < num_literals; ++l)
+    if (h_counts[4 * l + 2] > 0x7fffffffull || h_counts[4 * l + 3] > 0x7fffffffull)
+      throw Error(QF_ERR_COUNT_RANGE, "a binding count exceeds int32 (values are exact in uint64)");
+}
+
+}  // namespace qf
+
+extern "C" {
+
 int qf_evaluate_literals(qf_context* c, const qf_table* binding, int64_t num_positive, const qf_table* relations,
                          const int* arg_types, int num_relations, const qf_literal* literals, int num_literals,
                          const qf_eval_options* opts, uint64_t* h_counts, uint64_t* d_semi_bits) {
@@ -1834,53 +1897,41 @@ int qf_evaluate_literals(qf_context* c, const qf_table* binding, int64_t num_pos
     qf_eval_options o;
     qf_eval_options_default(&o);
     if (opts) o = *opts;
-    std::vector<int> type_start(num_relations + 1, 0);
-    for (int r = 0; r < num_relations; ++r) {
-      check_table(&relations[r], "relation");
-      QF_REQUIRE(relations[r].num_columns <= QF_MAX_ARITY, "relation arity above QF_MAX_ARITY");
-      type_start[r + 1] = type_start[r] + relations[r].num_columns;
-    }
-    // Group literals by (relation, join position, clause join variable)
-    // (learner/QuickFoil.cpp:142-161, CandidateLiteralEvaluator.cpp:281-289); PLAIN_LITERAL
-    // keeps one group per literal.
-    std::vector<GroupSpec> specs;
-    std::map<std::tuple<int, int, int>, int> index;
-    for (int l = 0; l < num_literals; ++l) {
-      const qf_literal& lit = literals[l];
-      QF_REQUIRE(lit.relation >= 0 && lit.relation < num_relations, "literal relation out of range");
-      const int arity = relations[lit.relation].num_columns;
-      const int* types = arg_types ? arg_types + type_start[lit.relation] : nullptr;
-      for (int i = 0; i < arity; ++i)
-        QF_REQUIRE(lit.vars[i] >= -1 && lit.vars[i] < binding->num_columns, "literal variable out of range");
-      const int jk = literal_join_key(lit, arity, types);
-      QF_REQUIRE(jk >= 0, "literal has no bound variable");
-      LitSpec ls;
-      ls.slot = l;
-      for (int i = 0; i < arity; ++i)
-        if (lit.vars[i] >= 0) {
-          ls.pcols.push_back(i);
-          ls.bcols.push_back(lit.vars[i]);
-        }
-      normalise_key(ls);
-      const auto key = std::make_tuple(lit.relation, jk, lit.vars[jk]);
-      int gi;
-      if (o.mode == QF_EVAL_PLAIN_LITERAL || !index.count(key)) {
-        gi = (int)specs.size();
-        GroupSpec g;
-        g.relation = lit.relation;
-        g.join_col = jk;
-        g.join_var = lit.vars[jk];
-        specs.push_back(g);
-        if (o.mode != QF_EVAL_PLAIN_LITERAL) index[key] = gi;
-      } else {
-        gi = index[key];
-      }
-      specs[gi].lits.push_back(ls);
-    }
+    std::vector<GroupSpec> specs =
+        literal_groups(binding->num_columns, relations, arg_types, num_relations, literals, num_literals, o);
     run_groups(ctx, *binding, num_positive, relations, specs, num_literals, o, h_counts, d_semi_bits);
-    for (int l = 0; l < num_literals; ++l)
-      if (h_counts[4 * l + 2] > 0x7fffffffull || h_counts[4 * l + 3] > 0x7fffffffull)
-        throw Error(QF_ERR_COUNT_RANGE, "a binding count exceeds int32 (values are exact in uint64)");
+    check_count_range(h_counts, num_literals);
+  });
+}
+
+int qf_evaluate_literals_split(qf_context* c, const qf_table* positives, const qf_table* negatives,
+                               const qf_table* relations, const int* arg_types, int num_relations,
+                               const qf_literal* literals, int num_literals, const qf_eval_options* opts,
+                               uint64_t* h_counts, uint64_t* d_semi_bits_pos, uint64_t* d_semi_bits_neg) {
+  return guarded(c, [&](Context* ctx) {
+    check_table(positives, "positive binding table");
+    check_table(negatives, "negative binding table");
+    QF_REQUIRE(positives->num_columns == negatives->num_columns, "the two binding tables differ in columns");
+    QF_REQUIRE(num_literals >= 0 && (num_literals == 0 || (literals && h_counts)), "null literals or counts");
+    QF_REQUIRE(num_relations >= 0 && (num_relations == 0 || relations), "null relations");
+    qf_eval_options o;
+    qf_eval_options_default(&o);
+    if (opts) o = *opts;
+    o.use_graph = 0;  // the two passes would replace each other's captured step
+    std::vector<GroupSpec> specs =
+        literal_groups(positives->num_columns, relations, arg_types, num_relations, literals, num_literals, o);
+    // ExecuteOnPositives, then ExecuteOnNegatives (CandidateLiteralEvaluator.cpp:345-405): every
+    // row of the first table is positive, none of the second.
+    std::vector<uint64_t> pos((size_t)std::max(1, num_literals) * 4), neg((size_t)std::max(1, num_literals) * 4);
+    run_groups(ctx, *positives, positives->num_rows, relations, specs, num_literals, o, pos.data(), d_semi_bits_pos);
+    run_groups(ctx, *negatives, 0, relations, specs, num_literals, o, neg.data(), d_semi_bits_neg);
+    for (int l = 0; l < num_literals; ++l) {
+      h_counts[4 * l + 0] = pos[4 * l + 0];
+      h_counts[4 * l + 1] = neg[4 * l + 1];
+      h_counts[4 * l + 2] = pos[4 * l + 2];
+      h_counts[4 * l + 3] = neg[4 * l + 3];
+    }
+    check_count_range(h_counts, num_literals);
   });
 }
 

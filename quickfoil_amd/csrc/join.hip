@@ -75,10 +75,12 @@ __global__ void k_semi_bits(Keys probe, int64_t n, Keys inner, const uint32_t* s
   }
 }
 
-// Per outer row: matching inner segment [seg_begin, seg_end) and its size.
+// Per outer row: matching inner segment [seg_begin, seg_end) and its size; counts[n] = 0, so
+// that an exclusive scan over n + 1 entries ends with the total.
 __global__ void k_match_counts(Keys outer, int64_t n, Keys inner, const uint32_t* state, uint32_t cap,
                                const int64_t* seg_offsets, int64_t* counts, int64_t* seg_end) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == n) counts[n] = 0;
   if (i >= n) return;
   const int32_t slot = keyset_find(outer, i, inner, state, cap);
   if (slot < 0) {
@@ -154,6 +156,63 @@ static KeySet build_keyset(Context* ctx, const Keys& k, int64_t n, const uint64_
   return s;
 }
 
+// The inner side of an ordered join: its key set, and its rows stably sorted by key slot (each
+// distinct key owns one segment of ascending rows).
+struct InnerSide {
+  Keys k{};
+  int64_t n = 0;
+  KeySet set;
+  DevBuf sorted_row, seg;
+};
+
+static void prepare_inner(Context* ctx, const std::vector<const int32_t*>& inner_keys, int64_t n_inner, InnerSide& in) {
+  in.k = make_keys(inner_keys);
+  in.n = n_inner;
+  if (n_inner == 0) return;
+  DevBuf slot_of(ctx, n_inner * 4);
+  in.set = build_keyset(ctx, in.k, n_inner, nullptr, slot_of.as<int32_t>());
+  DevBuf sorted_slot(ctx, n_inner * 4);
+  in.sorted_row = DevBuf(ctx, n_inner * 4);
+  in.seg = DevBuf(ctx, ((size_t)in.set.cap + 1) * 8);
+  const int32_t* src[1] = {slot_of.as<int32_t>()};
+  int32_t* out[2] = {sorted_slot.as<int32_t>(), in.sorted_row.as<int32_t>()};
+  stable_partition(ctx, src, 1, 0, true, n_inner, 0, log2_u32(in.set.cap), out, 1, in.seg.as<int64_t>());
+}
+
+// The outer side: per outer row the size and end of its inner segment, and the output offsets.
+struct OuterSide {
+  int64_t n = 0;
+  int64_t total = 0;
+  DevBuf counts, seg_end, out_off;
+};
+
+// Counts the output rows of outer x inner (one copy to the host: the scan's last entry).
+static int64_t count_outer(Context* ctx, const std::vector<const int32_t*>& outer_keys, int64_t n_outer,
+                           const InnerSide& in, OuterSide& o) {
+  o.n = n_outer;
+  o.total = 0;
+  if (n_outer == 0 || in.n == 0) return 0;
+  const Keys ko = make_keys(outer_keys);
+  o.counts = DevBuf(ctx, (n_outer + 1) * 8);
+  o.seg_end = DevBuf(ctx, n_outer * 8);
+  o.out_off = DevBuf(ctx, (n_outer + 1) * 8);
+  hipLaunchKernelGGL(k_match_counts, dim3((unsigned)ceil_div(n_outer + 1, 256)), dim3(256), 0, ctx->stream, ko,
+                     n_outer, in.k, in.set.state.as<uint32_t>(), in.set.cap, in.seg.as<int64_t>(),
+                     o.counts.as<int64_t>(), o.seg_end.as<int64_t>());
+  QF_CHECK_HIP(hipGetLastError());
+  scan_exclusive_i64(ctx, o.counts.as<int64_t>(), o.out_off.as<int64_t>(), n_outer + 1);
+  copy_d2h(ctx, &o.total, o.out_off.as<int64_t>() + n_outer, 8);
+  return o.total;
+}
+
+// Writes the joined rows (outer rows in order, equal-key inner rows in descending order).
+static void project(Context* ctx, const OuterSide& o, const InnerSide& in, const ProjectArgs& proj) {
+  if (o.total == 0) return;
+  hipLaunchKernelGGL(k_project, dim3((unsigned)ceil_div(o.total, 256)), dim3(256), 0, ctx->stream, proj,
+                     o.out_off.as<int64_t>(), o.seg_end.as<int64_t>(), in.sorted_row.as<int32_t>(), o.n, o.total);
+  QF_CHECK_HIP(hipGetLastError());
+}
+
 // Outer rows in order, inner equal-key rows in descending order; projections per output column.
 // Returns the row count; writes only when out != null and the count equals expect (expect < 0:
 // any count).
@@ -161,30 +220,12 @@ static int64_t ordered_join(Context* ctx, const std::vector<const int32_t*>& out
                             const std::vector<const int32_t*>& inner_keys, int64_t n_inner,
                             const ProjectArgs& proj, bool write, int64_t expect, int64_t capacity = -1) {
   if (n_outer == 0 || n_inner == 0) return 0;
-  const Keys ko = make_keys(outer_keys), ki = make_keys(inner_keys);
-  DevBuf slot_of(ctx, n_inner * 4);
-  KeySet set = build_keyset(ctx, ki, n_inner, nullptr, slot_of.as<int32_t>());
-  // Inner rows sorted by slot, ascending row within a slot; segment offsets per slot.
-  DevBuf sorted_slot(ctx, n_inner * 4), sorted_row(ctx, n_inner * 4), seg(ctx, ((size_t)set.cap + 1) * 8);
-  {
-    const int32_t* in[1] = {slot_of.as<int32_t>()};
-    int32_t* out[2] = {sorted_slot.as<int32_t>(), sorted_row.as<int32_t>()};
-    stable_partition(ctx, in, 1, 0, true, n_inner, 0, log2_u32(set.cap), out, 1, seg.as<int64_t>());
-  }
-  DevBuf counts(ctx, n_outer * 8), seg_end(ctx, n_outer * 8), out_off(ctx, n_outer * 8);
-  hipLaunchKernelGGL(k_match_counts, dim3((unsigned)ceil_div(n_outer, 256)), dim3(256), 0, ctx->stream, ko, n_outer,
-                     ki, set.state.as<uint32_t>(), set.cap, seg.as<int64_t>(), counts.as<int64_t>(),
-                     seg_end.as<int64_t>());
-  QF_CHECK_HIP(hipGetLastError());
-  scan_exclusive_i64(ctx, counts.as<int64_t>(), out_off.as<int64_t>(), n_outer);
-  int64_t last_off = 0, last_cnt = 0;
-  copy_d2h(ctx, &last_off, out_off.as<int64_t>() + n_outer - 1, 8);
-  copy_d2h(ctx, &last_cnt, counts.as<int64_t>() + n_outer - 1, 8);
-  const int64_t total = last_off + last_cnt;
+  InnerSide in;
+  prepare_inner(ctx, inner_keys, n_inner, in);
+  OuterSide o;
+  const int64_t total = count_outer(ctx, outer_keys, n_outer, in, o);
   if (!write || total == 0 || (expect >= 0 && total != expect) || (capacity >= 0 && total > capacity)) return total;
-  hipLaunchKernelGGL(k_project, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, ctx->stream, proj,
-                     out_off.as<int64_t>(), seg_end.as<int64_t>(), sorted_row.as<int32_t>(), n_outer, total);
-  QF_CHECK_HIP(hipGetLastError());
+  project(ctx, o, in, proj);
   sync(ctx);
   return total;
 }
@@ -228,38 +269,20 @@ int qf_create_label_aware_binding_table(qf_context* c, const qf_table* binding, 
     const int ncols = C + (int)unbound.size();
     QF_REQUIRE(ncols <= 16, "too many output columns");
     QF_REQUIRE(num_binding_pos >= 0 && num_binding_neg >= 0, "negative sizes");
-    // Which side is probed follows MultiColumnHashJoin.cpp:160-249.
+    // Which side is probed follows MultiColumnHashJoin.cpp:160-249.  Each section is joined
+    // once: both sizes are checked against the evaluator's counts before anything is written (the
+    // reference writes into its exact presize unchecked, :144-159).
     const bool bg_outer = npos < nbg && nneg < nbg;
+    InnerSide inner[2];
+    OuterSide outer[2];
+    ProjectArgs proj[2];
+    if (!bg_outer) prepare_inner(ctx, bg_keys, nbg, inner[0]);  // the relation is the inner side of both
     for (int label = 0; label < 2; ++label) {
       const int64_t nb = label == 0 ? npos : nneg;
       const int64_t base = label == 0 ? 0 : npos;
       const int64_t expect = label == 0 ? num_binding_pos : num_binding_neg;
-      ProjectArgs p{};
-      p.ncols = ncols;
-      for (int col = 0; col < ncols; ++col) {
-        p.out[col] = d_out_columns[col] + (label == 0 ? 0 : num_binding_pos);
-        const bool clause_col = col < C;
-        const int32_t* src = clause_col ? binding->columns[col] + base : relation->columns[unbound[col - C]];
-        const bool from_bg = !clause_col;
-        p.from_outer[col] = bg_outer ? from_bg : !from_bg;
-        if (p.from_outer[col]) p.outer_cols[col] = src;
-        else p.inner_cols[col] = src;
-      }
-      const std::vector<const int32_t*>& bkeys = label == 0 ? pos_keys : neg_keys;
-      int64_t total;
-      if (bg_outer) total = ordered_join(ctx, bg_keys, nbg, bkeys, nb, p, false, expect);
-      else total = ordered_join(ctx, bkeys, nb, bg_keys, nbg, p, false, expect);
-      if (total != expect)
-        throw Error(QF_ERR_OVERFLOW, "join produced " + std::to_string(total) + " rows for a " +
-                                         (label == 0 ? "positive" : "negative") + " section sized " +
-                                         std::to_string(expect));
-    }
-    // Sizes verified; write both sections.
-    for (int label = 0; label < 2; ++label) {
-      const int64_t nb = label == 0 ? npos : nneg;
-      const int64_t base = label == 0 ? 0 : npos;
-      const int64_t expect = label == 0 ? num_binding_pos : num_binding_neg;
-      ProjectArgs p{};
+      ProjectArgs& p = proj[label];
+      p = ProjectArgs{};
       p.ncols = ncols;
       for (int col = 0; col < ncols; ++col) {
         p.out[col] = d_out_columns[col] + (label == 0 ? 0 : num_binding_pos);
@@ -270,9 +293,23 @@ int qf_create_label_aware_binding_table(qf_context* c, const qf_table* binding, 
         else p.inner_cols[col] = src;
       }
       const std::vector<const int32_t*>& bkeys = label == 0 ? pos_keys : neg_keys;
-      if (bg_outer) ordered_join(ctx, bg_keys, nbg, bkeys, nb, p, true, expect);
-      else ordered_join(ctx, bkeys, nb, bg_keys, nbg, p, true, expect);
+      int64_t total = 0;
+      if (nb > 0 && nbg > 0) {
+        if (bg_outer) {
+          prepare_inner(ctx, bkeys, nb, inner[label]);
+          total = count_outer(ctx, bg_keys, nbg, inner[label], outer[label]);
+        } else {
+          total = count_outer(ctx, bkeys, nb, inner[0], outer[label]);
+        }
+      }
+      if (total != expect)
+        throw Error(QF_ERR_OVERFLOW, "join produced " + std::to_string(total) + " rows for a " +
+                                         (label == 0 ? "positive" : "negative") + " section sized " +
+                                         std::to_string(expect));
     }
+    for (int label = 0; label < 2; ++label)
+      if (outer[label].total > 0) project(ctx, outer[label], inner[bg_outer ? label : 0], proj[label]);
+    sync(ctx);
   });
 }
 

@@ -269,3 +269,22 @@ def test_arg_types_join_key_rule(qf_ctx, mode):
     got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, arg_types=arg_types, mode=mode, semi_bits=True)
     np.testing.assert_array_equal(got, want)
     np.testing.assert_array_equal(bits, want_bits)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("npos", [170, 400, 0])
+def test_split_positive_and_negative_tables(qf_ctx, mode, npos):
+    """Separate positive and negative binding tables (the restart clause's two passes,
+    learner/CandidateLiteralEvaluator.cpp:345-405) give the counts of the concatenated table, with
+    each table's covered bitvector over its own rows; a table may be empty."""
+    binding, _, rels, lits = random_problem(51, n=400, rel_shapes=((2, 300), (3, 200)), domain=12, zipf=True)
+    want, want_bits = oracle.Problem(binding, npos, rels, lits).bruteforce(semi=True)
+    pos = [_t(c[:npos]) for c in binding]
+    neg = [_t(c[npos:]) for c in binding]
+    got, bp, bn = qf_ctx.evaluate_literals_split(pos, neg, [[_t(c) for c in r] for r in rels], lits, mode=mode,
+                                                 semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    rows = np.unpackbits(want_bits.view(np.uint8), axis=1, bitorder="little")[:, :400]
+    for part, lo, hi in ((bp, 0, npos), (bn, npos, 400)):
+        got_rows = np.unpackbits(part.cpu().numpy().view(np.uint8), axis=1, bitorder="little")[:, : hi - lo]
+        np.testing.assert_array_equal(got_rows, rows[:, lo:hi])
