@@ -92,3 +92,50 @@ def brute_extended_join(binding, num_positive, relation, vars_):
                 row = tuple([int(binding[c][r]) for c in range(C)] + [int(relation[u][s]) for u in unbound])
                 (pos_rows if r < num_positive else neg_rows).append(row)
     return sorted(pos_rows), sorted(neg_rows)
+
+
+def build_abi_driver(out_dir):
+    """Compiles tests/abi_driver.cpp (a C++ caller of include/quickfoil_amd.hpp) with g++ against
+    the built library; returns the executable's path."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib_dir = os.path.join(root, "quickfoil_amd")
+    out = os.path.join(out_dir, "abi_driver")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-I" + os.path.join(root, "include"), os.path.join(root, "tests", "abi_driver.cpp"),
+                    "-L" + lib_dir, "-lquickfoil_amd", "-L/opt/rocm/lib", "-lamdhip64",
+                    "-Wl,-rpath," + lib_dir + ":/opt/rocm/lib", "-o", out], check=True)
+    return out
+
+
+def abi_driver_script(binding, num_positive, relations, literals, plans=()):
+    """The driver's stdin for a problem (see tests/abi_driver.cpp); plans: (relation, join column,
+    binding column, atoms [(bg, bind, literal)], conjunctions [(left, right, literal)], root)."""
+    lines = ["BINDING %d %d %d" % (len(binding), len(binding[0]), num_positive)]
+    lines += [" ".join(map(str, c.tolist())) for c in binding]
+    for r in relations:
+        lines.append("REL %d %d" % (len(r), len(r[0])))
+        lines += [" ".join(map(str, c.tolist())) for c in r]
+    for rel, vars_ in literals:
+        lines.append("LIT %d %s" % (rel, " ".join(map(str, list(vars_) + [-1] * (6 - len(vars_))))))
+    for rel, jc, bc, atoms, conj, root in plans:
+        lines.append("PLAN %d %d %d %d %d %d" % (rel, jc, bc, len(atoms), len(conj), root))
+        lines += ["%d %d %d" % a for a in atoms]
+        lines += ["%d %d %d" % c for c in conj]
+    lines.append("END")
+    return "\n".join(lines) + "\n"
+
+
+def parse_driver_output(text):
+    """{tag: {index: [values]}} for counted lines, {tag: [values]} for arrays."""
+    out = {}
+    for line in text.splitlines():
+        parts = line.split()
+        tag = parts[0]
+        if tag in ("EVAL", "SPLIT", "AGG", "AGG2"):
+            out.setdefault(tag, {})[int(parts[1])] = [int(x) for x in parts[2:]]
+        elif tag == "SIZE":
+            out.setdefault(tag, {})[int(parts[1])] = parts[2]
+        else:
+            out[tag] = [int(x) for x in parts[1:]]
+    return out

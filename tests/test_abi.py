@@ -42,3 +42,11 @@ def test_bucket_layout_host_helper():
     # P2(3)=4, empty -> 0, P2(5)=8, P2(1)=1 (storage/FoilHashTable.cpp:34-49).
     assert total == 13
     assert boff.tolist() == [0, 4, 4, 12, 13]
+
+
+def test_cpp_adapter_driver_compiles_and_links(tmp_path):
+    """tests/abi_driver.cpp, a C++ caller of the adapter header include/quickfoil_amd.hpp (the
+    reference's operator names over the C ABI), compiles and links against the library."""
+    from helpers import build_abi_driver
+    path = build_abi_driver(str(tmp_path))
+    assert os.path.exists(path)
