@@ -480,19 +480,31 @@ def bench_learner(args):
 
 
 def bench_estimator(args):
-    """c5: record join instances from refinement steps (1/64 scale, varied seeds), train the MLP,
-    report q-error on held-out steps, presize fallback rate and join time against two passes."""
+    """c5: train the cardinality MLP on joins recorded from refinement steps (1/64 scale, varied
+    seeds) and operator microbenches, report its q-error on held-out steps beside the textbook
+    independence estimate, then use it in the loop at the full refinement step: CreateBindingTable
+    joins of the step's literals presized by the predictions in one call (exact fallback for
+    under-predictions) against the two-pass joins, outputs asserted identical."""
     import torch
     import quickfoil_amd
-    from quickfoil_amd import estimator
+    from quickfoil_amd import datagen, estimator
     ctx = quickfoil_amd.Context(0)
     t0 = time.perf_counter()
-    train = [estimator.record_step(ctx, 1.0 / 64, 100 + i, log=log)[:2] for i in range(24)]
+    # Steps at four scales (1/64 to the full 8,388,608-row step, six seeds each: 49,920 joins) and
+    # operator microbenches at two scales and both key distributions.
+    scales = [1.0 / 64, 1.0 / 16, 1.0 / 4, 1.0]
+    train = [estimator.record_step(ctx, sc, 100 + 10 * k + i, log=log, on_device=True)[:2]
+             for k, sc in enumerate(scales) for i in range(6)]
+    train += [estimator.record_micro(ctx, sc, 0x51F00000 + 97 * i, d, log=log)
+              for i, sc in enumerate([1.0 / 64, 1.0 / 16] * 4) for d in ("uniform", "zipf")]
     X = np.concatenate([t[0] for t in train])
     y = np.concatenate([t[1] for t in train])
-    held = [estimator.record_step(ctx, 1.0 / 64, 900 + i, log=log) for i in range(2)]
+    torch.cuda.empty_cache()
+    held = [estimator.record_step(ctx, sc, 900 + i, log=log, on_device=True)[:2]
+            for i, sc in enumerate([1.0 / 64, 1.0 / 4])]
     Xh = np.concatenate([h[0] for h in held])
     yh = np.concatenate([h[1] for h in held])
+    torch.cuda.empty_cache()
     rec_s = time.perf_counter() - t0
     est = estimator.Estimator(ctx)
     torch.cuda.synchronize()
@@ -500,51 +512,70 @@ def bench_estimator(args):
     est.fit(X, y, epochs=60, batch=512, lr=3e-3, log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    pred = est.predict(Xh)
-    q = estimator.q_error(pred, yh)
+    q = estimator.q_error(est.predict(Xh), yh)
     qi = estimator.q_error(estimator.independence_estimate(Xh), yh)
-    # Presized joins on the first held-out step against the two-pass join.
-    _, _, w = held[0]
-    dev = torch.device("cuda")
-    binding = [torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in w["binding"]]
-    rels = [[torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in r] for r in w["relations"]]
-    picks = [i for i in range(len(w["literals"])) if yh[i] < 24][:: max(1, len(w["literals"]) // 300)]
-    fallbacks, identical = 0, True
-    # Output columns come from one preallocated arena in both forms, so that the comparison is
-    # the engine's work (a count pass saved) and not the allocator's.
-    arena_rows = max([estimator.presize_rows(pred[i]) for i in picks] + [int(2.0 ** yh[i]) + 1 for i in picks])
-    arena = [torch.empty(arena_rows, dtype=torch.int32, device=dev) for _ in range(len(binding) + 3)]
+
+    # In the loop, full refinement step: the step's exact counts (one batched evaluation) pick
+    # which literals' binding tables to build and judge the predictions; the joins themselves
+    # see only the estimator.
+    dev = torch.device("cuda:0")
+    w = datagen.refinement_step(device=dev)
+    lits = w["literals"]
+    counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], lits, reuse_partitions=True)
+    rows_of = counts[:, 2].astype(np.int64) + counts[:, 3].astype(np.int64)
+    mid = [i for i in range(len(lits)) if (1 << 10) <= rows_of[i] <= (1 << 22)]
+    wide = [i for i in range(len(lits)) if sum(1 for v in lits[i][1] if v >= 0) >= 2]
+    picks = mid[:: max(1, len(mid) // 36)][:36] + wide[:: max(1, len(wide) // 12)][:12]
+    joins = [lits[i] for i in picks]
+    log("in-loop joins: %d literals, %d output rows" % (len(joins), int(rows_of[picks].sum())))
+    binding, rels = w["binding"], w["relations"]
+    # Warm both code paths.
+    estimator.presized_joins(ctx, est, binding, rels, joins[:2])
+    ctx.create_binding_table(binding, rels[joins[0][0]], joins[0][1])
     torch.cuda.synchronize()
+    arena_rows = int(max(1, rows_of[picks].max()))
+    arena = [torch.empty(arena_rows, dtype=torch.int32, device=dev) for _ in range(len(binding) + 3)]
     t0 = time.perf_counter()
-    for i in picks:
-        rel, vars_ = w["literals"][i]
+    for rel, vars_ in joins:
         ctx.create_binding_table(binding, rels[rel], vars_, out=arena)
     torch.cuda.synchronize()
     two_pass_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    for i in picks:
-        rel, vars_ = w["literals"][i]
-        _, fb = ctx.create_binding_table_presized(binding, rels[rel], vars_, estimator.presize_rows(pred[i]),
-                                                  out=arena)
-        fallbacks += int(fb)
+    got, stats = estimator.presized_joins(ctx, est, binding, rels, joins)
     torch.cuda.synchronize()
-    presized_s = time.perf_counter() - t0
-    for i in picks[:: max(1, len(picks) // 20)]:
-        rel, vars_ = w["literals"][i]
-        a = ctx.create_binding_table(binding, rels[rel], vars_)
-        b, _ = ctx.create_binding_table_presized(binding, rels[rel], vars_, estimator.presize_rows(pred[i]))
-        identical = identical and all(torch.equal(x, z) for x, z in zip(a, b))
-    out = {"metric": "join-cardinality estimator q-error (held-out joins)", "value": float(np.median(q)),
-           "unit": "q-error p50", "n_gpus": 1, "higher_is_better": False, "dtype": "bf16 (fp32 accumulate)",
-           "data": "synthetic (joins recorded from refinement steps at 1/64 scale)",
-           "config": {"workload": "estimator: MLP 40-64-64-1, %d training joins from 24 steps, %d held-out"
-                                  % (len(y), len(yh))},
-           "q_error_p50": float(np.median(q)), "q_error_p99": float(np.percentile(q, 99)),
-           "independence_q_error_p50": float(np.median(qi)), "independence_q_error_p99": float(np.percentile(qi, 99)),
-           "presized_joins": len(picks), "fallback_rate": fallbacks / max(1, len(picks)),
-           "two_pass_join_s": two_pass_s, "presized_join_s": presized_s,
-           "presize_speedup": two_pass_s / presized_s if presized_s > 0 else None,
-           "identical_outputs": identical, "record_s": rec_s, "train_s": train_s}
+    assisted_s = time.perf_counter() - t0
+    identical = True
+    for (rel, vars_), g in zip(joins, got):
+        want = ctx.create_binding_table(binding, rels[rel], vars_)
+        identical = identical and len(g) == len(want) and all(torch.equal(a, b) for a, b in zip(g, want))
+    assert identical, "presized joins differ from the two-pass joins"
+    assert np.array_equal(stats["rows"], rows_of[picks]), "join sizes differ from the evaluator's counts"
+    y_loop = np.log2(1.0 + rows_of[picks].astype(np.float64))
+    q_loop = estimator.q_error(stats["predicted_log2"], y_loop)
+    q_loop_ind = estimator.q_error(estimator.independence_estimate(
+        estimator.binding_join_features(binding, rels, joins)), y_loop)
+    out = {"metric": "join-cardinality estimator q-error (held-out joins) and in-loop presized joins",
+           "value": float(np.median(q)), "unit": "q-error p50", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "bf16 (fp32 accumulate)",
+           "data": "synthetic (joins recorded from refinement steps and operator microbenches; in-loop joins on "
+                   "the full refinement step, whose seed no training step uses)",
+           "config": {"workload": "estimator: MLP 40-64-64-1, %d training joins (24 refinement steps at 1/64 to full "
+                                  "scale, 16 microbenches), %d held-out (steps at 1/64 and 1/4 scale); %d in-loop "
+                                  "CreateBindingTable joins of the full refinement step"
+                                  % (len(y), len(yh), len(joins))},
+           "q_error_p50": float(np.median(q)), "q_error_p90": float(np.percentile(q, 90)),
+           "q_error_p99": float(np.percentile(q, 99)),
+           "independence_q_error_p50": float(np.median(qi)), "independence_q_error_p90": float(np.percentile(qi, 90)),
+           "independence_q_error_p99": float(np.percentile(qi, 99)),
+           "in_loop": {"joins": len(joins), "output_rows": int(rows_of[picks].sum()),
+                       "fallbacks": stats["fallbacks"], "fallback_rate": stats["fallbacks"] / max(1, len(joins)),
+                       "q_error_p50": float(np.median(q_loop)), "q_error_p99": float(np.percentile(q_loop, 99)),
+                       "independence_q_error_p50": float(np.median(q_loop_ind)),
+                       "independence_q_error_p99": float(np.percentile(q_loop_ind, 99)),
+                       "two_pass_s": two_pass_s, "assisted_s": assisted_s,
+                       "speedup": two_pass_s / assisted_s if assisted_s > 0 else None,
+                       "identical_outputs": identical},
+           "record_s": rec_s, "train_s": train_s}
     print(json.dumps(out), flush=True)
     est.close()
     ctx.close()

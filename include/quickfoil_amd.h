@@ -236,6 +236,20 @@ int qf_create_binding_table_presized(qf_context* ctx, const qf_table* binding, c
                                      const qf_literal* literal, int64_t capacity, int32_t* const* d_out_columns,
                                      int64_t* h_num_rows);
 
+/* CreateBindingTable for several literals over one binding table, into caller-presized outputs
+ * (capacities from an estimate: the learned estimator below), without a host round trip per
+ * join: each join counts its rows on the device and writes them only when they fit.  No
+ * counterpart in the reference, whose buffers grow by 1.5x (operations/MultiColumnHashJoin.cpp:
+ * 445-461).  d_out_columns[i]: literal i's output column pointers (binding columns, then one per
+ * new variable, as qf_create_binding_table), each of capacities[i] rows.  h_num_rows[i] receives
+ * the exact row count; h_fits[i] = 1 when the rows were written, 0 when the capacity was too
+ * small and nothing was written (the exact fallback: call again for those literals with their
+ * exact counts).  Returns QF_ERR_OVERFLOW when any literal did not fit. */
+int qf_create_binding_tables_presized(qf_context* ctx, const qf_table* binding, const qf_table* relations,
+                                      int num_relations, const qf_literal* literals, int num_literals,
+                                      const int64_t* capacities, int32_t* const* const* d_out_columns,
+                                      int64_t* h_num_rows, uint8_t* h_fits);
+
 /* LeftSemiJoin::Next over all chunks (operations/LeftSemiJoin.hpp:56-141): bit i of
  * d_out_bits (ceil(probe rows/64) words) = probe row i has a build row with equal keys.
  * probe_keys and build_keys hold the key columns (same count, 1..QF_MAX_ARITY). */

@@ -124,6 +124,8 @@ SIGNATURES = {
                                                  ctypes.POINTER(QfLiteral), _I64, _I64, _P]),
     "qf_create_binding_table": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable),
                                      ctypes.POINTER(QfLiteral), _P, ctypes.POINTER(_I64)]),
+    "qf_create_binding_tables_presized": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), _I,
+                                               ctypes.POINTER(QfLiteral), _I, _P, _P, _P, _P]),
     "qf_left_semi_join": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_right_semi_join": (_I, [_P, ctypes.POINTER(QfTable), ctypes.POINTER(QfTable), _P, ctypes.POINTER(_I64)]),
     "qf_build_hash_table_after_semi_join": (_I, [_P, ctypes.POINTER(QfTable), _P, _P, _P, _P]),
@@ -459,6 +461,42 @@ class Context:
         self._check(self.lib.qf_create_binding_table(self.h, ctypes.byref(b.t), ctypes.byref(r.t), ctypes.byref(lit),
                                                      arr, ctypes.byref(rows)))
         return [o[: rows.value] for o in outs]
+
+    def create_binding_tables_presized(self, binding_columns, relations, literals, capacities):
+        """CreateBindingTable of every literal ((relation, vars) over `relations`) into outputs of
+        the given row capacities, in one call.  Returns (outputs: per literal its columns, or None
+        when its rows exceeded the capacity and nothing was written; exact row counts; fits)."""
+        t = self.torch
+        b = _TableHolder(binding_columns)
+        rels = [_TableHolder(r) for r in relations]
+        rel_arr = (QfTable * max(1, len(rels)))(*[r.t for r in rels])
+        lits = LiteralArray(literals)
+        C = len(binding_columns)
+        ncols = [C + sum(1 for v in vars_ if v < 0 or v >= C) for _, vars_ in literals]
+        caps = np.ascontiguousarray(np.asarray(capacities, dtype=np.int64).reshape(-1))
+        assert caps.size == len(literals) and (caps >= 0).all()
+        total = int(sum(max(1, int(c)) * n for c, n in zip(caps, ncols)))
+        arena = t.empty(max(1, total), dtype=t.int32, device=binding_columns[0].device)
+        cols, col_arrays, at = [], [], 0
+        for c, n in zip(caps, ncols):
+            rows = max(1, int(c))
+            these = [arena[at + k * rows: at + (k + 1) * rows] for k in range(n)]
+            at += n * rows
+            cols.append(these)
+            col_arrays.append((ctypes.c_void_p * n)(*[x.data_ptr() for x in these]))
+        ptrs = (ctypes.c_void_p * max(1, len(col_arrays)))(*[ctypes.cast(a, ctypes.c_void_p).value
+                                                             for a in col_arrays])
+        out_rows = np.zeros(max(1, len(literals)), dtype=np.int64)
+        fits = np.zeros(max(1, len(literals)), dtype=np.uint8)
+        lit_arr = lits.pointer() if len(literals) else (QfLiteral * 1)()
+        st = self.lib.qf_create_binding_tables_presized(self.h, ctypes.byref(b.t), rel_arr, len(rels), lit_arr,
+                                                        len(literals), caps.ctypes.data_as(ctypes.c_void_p), ptrs,
+                                                        out_rows.ctypes.data_as(ctypes.c_void_p),
+                                                        fits.ctypes.data_as(ctypes.c_void_p))
+        if st not in (QF_OK, 4):
+            self._check(st)
+        outs = [[x[: int(r)] for x in these] if f else None for these, r, f in zip(cols, out_rows, fits)]
+        return outs[: len(literals)], out_rows[: len(literals)], fits[: len(literals)].astype(bool)
 
     def create_binding_table_presized(self, binding_columns, relation_columns, vars_, capacity, out=None):
         """CreateBindingTable into buffers of `capacity` rows (the first `capacity` rows of `out`

@@ -117,6 +117,27 @@ __global__ void k_project(ProjectArgs p, const int64_t* out_off, const int64_t* 
   for (int c = 0; c < p.ncols; ++c) p.out[c][e] = p.from_outer[c] ? p.outer_cols[c][o] : p.inner_cols[c][inner];
 }
 
+// k_project for a caller-presized output: the row count is read on the device (out_off[n_outer],
+// the scan's last entry) and recorded in *total_out; the rows are written only when they fit
+// `capacity`, otherwise nothing is (the caller retries with the exact count).
+__global__ void k_project_presized(ProjectArgs p, const int64_t* out_off, const int64_t* seg_end,
+                                   const int32_t* inner_sorted, int64_t n_outer, int64_t capacity,
+                                   int64_t* total_out) {
+  const int64_t total = out_off[n_outer];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) *total_out = total;
+  if (total > capacity || e >= total) return;
+  int64_t lo = 0, hi = n_outer - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (out_off[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const int64_t k = e - out_off[lo];
+  const int64_t inner = inner_sorted[seg_end[lo] - 1 - k];
+  for (int c = 0; c < p.ncols; ++c) p.out[c][e] = p.from_outer[c] ? p.outer_cols[c][lo] : p.inner_cols[c][inner];
+}
+
 __global__ void k_popcount_total(const uint64_t* bits, int64_t words, unsigned long long* ones) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t c = i < words ? (uint32_t)__popcll(bits[i]) : 0u;
@@ -186,9 +207,10 @@ struct OuterSide {
   DevBuf counts, seg_end, out_off;
 };
 
-// Counts the output rows of outer x inner (one copy to the host: the scan's last entry).
+// Counts the output rows of outer x inner: the scan's last entry, copied to the host unless
+// `to_host` is false (the count then stays on the device at out_off[n_outer]).
 static int64_t count_outer(Context* ctx, const std::vector<const int32_t*>& outer_keys, int64_t n_outer,
-                           const InnerSide& in, OuterSide& o) {
+                           const InnerSide& in, OuterSide& o, bool to_host = true) {
   o.n = n_outer;
   o.total = 0;
   if (n_outer == 0 || in.n == 0) return 0;
@@ -201,7 +223,7 @@ static int64_t count_outer(Context* ctx, const std::vector<const int32_t*>& oute
                      o.counts.as<int64_t>(), o.seg_end.as<int64_t>());
   QF_CHECK_HIP(hipGetLastError());
   scan_exclusive_i64(ctx, o.counts.as<int64_t>(), o.out_off.as<int64_t>(), n_outer + 1);
-  copy_d2h(ctx, &o.total, o.out_off.as<int64_t>() + n_outer, 8);
+  if (to_host) copy_d2h(ctx, &o.total, o.out_off.as<int64_t>() + n_outer, 8);
   return o.total;
 }
 
@@ -377,6 +399,83 @@ int qf_create_binding_table_presized(qf_context* c, const qf_table* binding, con
                                      int64_t* h_num_rows) {
   if (capacity < 0 || !d_out_columns) return QF_ERR_INVALID;
   return create_binding_table(c, binding, relation, literal, d_out_columns, h_num_rows, capacity);
+}
+
+int qf_create_binding_tables_presized(qf_context* c, const qf_table* binding, const qf_table* relations,
+                                      int num_relations, const qf_literal* literals, int num_literals,
+                                      const int64_t* capacities, int32_t* const* const* d_out_columns,
+                                      int64_t* h_num_rows, uint8_t* h_fits) {
+  bool any_over = false;
+  const int st = guarded(c, [&](Context* ctx) {
+    check_tab(binding, "binding table");
+    QF_REQUIRE(num_literals >= 0 && num_relations >= 0, "negative counts");
+    QF_REQUIRE(num_literals == 0 || (literals && capacities && d_out_columns && h_num_rows && h_fits),
+               "null literal arguments");
+    const int C = binding->num_columns;
+    const int64_t N = binding->num_rows;
+    // Every join is enqueued before any result is read.  A join's temporaries go back to the
+    // pool after its launches: later work on the stream reuses them only after those finish.
+    DevBuf totals(ctx, (size_t)std::max(1, num_literals) * 8);
+    fill_i64(ctx, totals.as<int64_t>(), 0, std::max(1, num_literals));
+    for (int i = 0; i < num_literals; ++i) {
+      const qf_literal& lit = literals[i];
+      QF_REQUIRE(lit.relation >= 0 && lit.relation < num_relations, "literal relation out of range");
+      const qf_table& rel = relations[lit.relation];
+      check_tab(&rel, "relation");
+      QF_REQUIRE(rel.num_columns <= QF_MAX_ARITY, "arity above QF_MAX_ARITY");
+      QF_REQUIRE(capacities[i] >= 0 && d_out_columns[i], "bad capacity or outputs");
+      std::vector<const int32_t*> lit_keys, bind_keys;
+      std::vector<int> unbound;
+      for (int k = 0; k < rel.num_columns; ++k) {
+        const int v = lit.vars[k];
+        if (v >= 0 && v < C) {  // CreateBindingTable: ids below the column count are join keys
+          lit_keys.push_back(rel.columns[k]);
+          bind_keys.push_back(binding->columns[v]);
+        } else {
+          unbound.push_back(k);
+        }
+      }
+      QF_REQUIRE(!lit_keys.empty(), "literal has no bound variable");
+      const int ncols = C + (int)unbound.size();
+      QF_REQUIRE(ncols <= 16, "too many output columns");
+      const int64_t nbg = rel.num_rows;
+      const bool binding_outer = nbg < N;  // MultiColumnHashJoin.cpp:55-98
+      ProjectArgs p{};
+      p.ncols = ncols;
+      for (int col = 0; col < ncols; ++col) {
+        p.out[col] = d_out_columns[i][col];
+        const bool clause_col = col < C;
+        const int32_t* src = clause_col ? binding->columns[col] : rel.columns[unbound[col - C]];
+        p.from_outer[col] = binding_outer ? clause_col : !clause_col;
+        if (p.from_outer[col]) p.outer_cols[col] = src;
+        else p.inner_cols[col] = src;
+      }
+      if (N == 0 || nbg == 0) continue;  // no rows; its total stays 0
+      InnerSide in;
+      OuterSide out;
+      if (binding_outer) {
+        prepare_inner(ctx, lit_keys, nbg, in);
+        count_outer(ctx, bind_keys, N, in, out, false);
+      } else {
+        prepare_inner(ctx, bind_keys, N, in);
+        count_outer(ctx, lit_keys, nbg, in, out, false);
+      }
+      hipLaunchKernelGGL(k_project_presized, dim3((unsigned)std::max<int64_t>(1, ceil_div(capacities[i], 256))),
+                         dim3(256), 0, ctx->stream, p, out.out_off.as<int64_t>(), out.seg_end.as<int64_t>(),
+                         in.sorted_row.as<int32_t>(), out.n, capacities[i], totals.as<int64_t>() + i);
+      QF_CHECK_HIP(hipGetLastError());
+    }
+    if (num_literals > 0) copy_d2h(ctx, h_num_rows, totals.get(), (size_t)num_literals * 8);
+    for (int i = 0; i < num_literals; ++i) {
+      h_fits[i] = h_num_rows[i] <= capacities[i] ? 1 : 0;
+      any_over = any_over || !h_fits[i];
+    }
+  });
+  if (st == QF_OK && any_over) {
+    reinterpret_cast<Context*>(c)->last_error = "some joins exceed their capacity (see h_fits)";
+    return QF_ERR_OVERFLOW;
+  }
+  return st;
 }
 
 static int semi_join(qf_context* c, const qf_table* probe_keys, const qf_table* build_keys, uint64_t* d_out_bits,

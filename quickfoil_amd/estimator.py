@@ -3,9 +3,16 @@ joins, batched inference, and presizing of binding-table joins with exact fallba
 
 A join instance is a binding table joined with one literal: the output has
 binding_pos + binding_neg rows (the evaluator's exact counts give the training targets).
-Features (40): log2 sizes and distinct-key estimates of both sides, arity, new-variable and key
-counts, the independence estimate log2(N_b N_p / max(D_b, D_p)), and 16-bin key-frequency
-histograms of a row sample of each side.
+Features (40): log2 sizes and composite distinct-key estimates of both sides, arity, new-variable
+and key counts, the independence estimate, and 16-bin key-frequency histograms of a row sample of
+each side.  The independence estimate is the textbook one for equi-joins under uniform,
+independent columns: N_b N_p / prod over key columns k of max(D_b(k), D_p(k)), with per-column
+distinct counts D estimated from the samples.
+
+In the loop (presized_joins): the network's predictions size the outputs of CreateBindingTable
+joins (the joins whose size no count gives beforehand), which run in one call without a host round
+trip per join; a join whose rows exceed its capacity writes nothing and is rerun at its exact size,
+so every output is exact.
 """
 import ctypes
 import math
@@ -18,11 +25,25 @@ NUM_FEATURES = 40
 SAMPLE = 1024
 
 
+def _rows(c):
+    return int(c.numel()) if hasattr(c, "numel") else int(np.asarray(c).size)
+
+
+def _host(c):
+    return c.cpu().numpy() if hasattr(c, "cpu") else np.asarray(c)
+
+
 def _sample(cols, sample):
-    n = int(cols[0].size)
+    """An evenly spaced row sample of the columns (numpy arrays or device tensors: only the
+    sampled rows leave the device)."""
+    n = _rows(cols[0])
     if n <= sample:
-        return [np.asarray(c) for c in cols], n
+        return [_host(c) for c in cols], n
     idx = np.linspace(0, n - 1, sample).astype(np.int64)
+    if hasattr(cols[0], "cpu"):  # a torch tensor
+        import torch
+        ti = torch.from_numpy(idx).to(cols[0].device)
+        return [c[ti].cpu().numpy() for c in cols], n
     return [np.asarray(c)[idx] for c in cols], n
 
 
@@ -44,8 +65,11 @@ def _key_stats(cols, sample=SAMPLE):
     return float(distinct), hist
 
 
-def join_features(binding_keys, relation_keys, arity, num_new, cache=None, cache_keys=(None, None)):
-    """Features of joining binding rows (key columns) with relation rows (key columns)."""
+def join_features(binding_keys, relation_keys, arity, num_new, cache=None, cache_keys=(None, None),
+                  column_keys=None):
+    """Features of joining binding rows (key columns) with relation rows (key columns).
+    cache_keys name the two composite keys, column_keys the (binding, relation) name of each key
+    column, for caching their sample statistics across joins."""
     def stats(cols, ck):
         if cache is not None and ck is not None and ck in cache:
             return cache[ck]
@@ -53,17 +77,25 @@ def join_features(binding_keys, relation_keys, arity, num_new, cache=None, cache
         if cache is not None and ck is not None:
             cache[ck] = v
         return v
-    nb, nr = int(binding_keys[0].size), int(relation_keys[0].size)
+    nb, nr = _rows(binding_keys[0]), _rows(relation_keys[0])
     db, hb = stats(binding_keys, cache_keys[0])
     dr, hr = stats(relation_keys, cache_keys[1])
     lb, lr = math.log2(1 + nb), math.log2(1 + nr)
-    ind = lb + lr - math.log2(1 + max(db, dr, 1.0))
+    # Independence: N_b N_p / prod_k max(D_b(k), D_p(k)) over the key columns.
+    log_rows = math.log2(max(1, nb)) + math.log2(max(1, nr))
+    for k in range(len(binding_keys)):
+        ckb, ckr = column_keys[k] if column_keys is not None else (None, None)
+        d_b = stats([binding_keys[k]], ckb)[0]
+        d_r = stats([relation_keys[k]], ckr)[0]
+        log_rows -= math.log2(max(1.0, d_b, d_r))
+    ind = math.log2(1.0 + 2.0 ** min(62.0, log_rows)) if nb and nr else 0.0
     f = [lb, lr, math.log2(1 + db), math.log2(1 + dr), arity, num_new, len(binding_keys), ind]
     return np.concatenate([np.array(f), hb, hr]).astype(np.float32)
 
 
 def independence_estimate(features):
-    """The estimator without the network: log2 output rows under uniform, independent keys."""
+    """The estimator without the network: log2(1 + output rows) under uniform, independent key
+    columns (feature 7)."""
     return np.maximum(0.0, features[:, 7])
 
 
@@ -76,7 +108,8 @@ def literal_instances(binding, relations, literals, counts, cache, tag):
         rkeys = [relations[rel][i] for i in bound]
         num_new = len(vars_) - len(bound)
         ck = ((tag, "b", tuple(vars_[i] for i in bound)), (tag, "r", rel, tuple(bound)))
-        X.append(join_features(bkeys, rkeys, len(vars_), num_new, cache, ck))
+        col_keys = [((tag, "b", (vars_[i],)), (tag, "r", rel, (i,))) for i in bound]
+        X.append(join_features(bkeys, rkeys, len(vars_), num_new, cache, ck, col_keys))
         y.append(math.log2(1 + float(c[2]) + float(c[3])))
     return np.stack(X), np.array(y, dtype=np.float32)
 
@@ -175,19 +208,98 @@ def presize_rows(pred_log2, margin=1.0):
     return int(min(2 ** 31 - 1, max(16, math.ceil(2.0 ** (float(pred_log2) + margin)))))
 
 
-def record_step(ctx, scale, seed, num_values=None, log=None):
+def record_step(ctx, scale, seed, num_values=None, log=None, on_device=False):
     """Join instances of one refinement step (every literal of the step workload at `scale`,
-    keys from `seed`): evaluates the step on the device and returns (features, targets)."""
+    keys from `seed`): evaluates the step on the device and returns (features, targets, workload).
+    on_device: the workload is generated on the GPU and stays there (features read row samples)."""
     import torch
     from . import datagen
     kw = {} if num_values is None else {"num_values": num_values}
     specs = datagen.step_relation_specs(seed=seed + 3)
-    w = datagen.refinement_step(scale=scale, seed=seed, specs=specs, **kw)
     dev = torch.device("cuda")
-    b = [torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in w["binding"]]
-    r = [[torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in rel] for rel in w["relations"]]
+    w = datagen.refinement_step(scale=scale, seed=seed, specs=specs, device=dev if on_device else None, **kw)
+    if on_device:
+        b, r = w["binding"], w["relations"]
+    else:
+        b = [torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in w["binding"]]
+        r = [[torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in rel] for rel in w["relations"]]
     counts, _ = ctx.evaluate_literals(b, w["num_positive"], r, w["literals"])
     X, y = literal_instances(w["binding"], w["relations"], w["literals"], counts, {}, seed)
     if log:
         log("recorded %d joins (scale %g, seed %d)" % (len(y), scale, seed))
     return X, y, w
+
+
+def record_micro(ctx, scale, seed, distribution, log=None):
+    """Join instances of the operator microbench at `scale` (one relation of arity 2 against three
+    binding variables, every literal placement), with the counts of the device evaluator."""
+    import torch
+    from . import datagen
+    rows = max(1024, int(datagen.MICRO_ROWS * scale))
+    w = datagen.microbench(distribution, rows=rows, rel_rows=max(256, int(datagen.MICRO_REL_ROWS * scale)),
+                           num_values=max(64, int(datagen.MICRO_KEYS * scale)), seed=seed)
+    literals = [(0, v) for v in datagen.all_placements(2, 3)]
+    dev = torch.device("cuda")
+    b = [torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in w["binding"]]
+    r = [[torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in rel] for rel in w["relations"]]
+    counts, _ = ctx.evaluate_literals(b, w["num_positive"], r, literals)
+    X, y = literal_instances(w["binding"], w["relations"], literals, counts, {}, ("micro", seed))
+    if log:
+        log("recorded %d joins (microbench %s, scale %g, seed %d)" % (len(y), distribution, scale, seed))
+    return X, y
+
+
+def binding_join_features(binding, relations, literals, cache=None):
+    """Features of CreateBindingTable joins (operations/MultiColumnHashJoin.cpp:26-115): binding
+    table x literal, where variables below the binding's column count are join keys and the
+    others new columns.  Columns may be device tensors (only samples are copied)."""
+    C = len(binding)
+    cache = {} if cache is None else cache
+    X = []
+    for rel, vars_ in literals:
+        bound = [i for i, v in enumerate(vars_) if 0 <= v < C]
+        bkeys = [binding[vars_[i]] for i in bound]
+        rkeys = [relations[rel][i] for i in bound]
+        ck = (("b", tuple(vars_[i] for i in bound)), ("r", rel, tuple(bound)))
+        col_keys = [(("b", (vars_[i],)), ("r", rel, (i,))) for i in bound]
+        X.append(join_features(bkeys, rkeys, len(vars_), len(vars_) - len(bound), cache, ck, col_keys))
+    return np.stack(X) if X else np.zeros((0, NUM_FEATURES), np.float32)
+
+
+def presized_joins(ctx, est, binding, relations, literals, margin=1.0, cache=None, budget_bytes=2 << 30):
+    """CreateBindingTable for every literal with the estimator in the loop: predicted sizes give
+    the capacities (2^(pred + margin) rows, at most the product of the input sizes), the joins run
+    largest prediction first, as few calls as the output budget allows, and those whose rows
+    exceed their capacity are rerun at their exact size.  Returns (outputs in literal order,
+    stats)."""
+    X = binding_join_features(binding, relations, literals, cache)
+    pred = est.predict(X) if len(literals) else np.zeros(0, np.float32)
+    C = len(binding)
+    nb = _rows(binding[0])
+    ncols = [C + sum(1 for v in vars_ if v < 0 or v >= C) for _, vars_ in literals]
+    caps = [min(presize_rows(p, margin), max(16, nb * _rows(relations[rel][0])), max(16, budget_bytes // (8 * n)))
+            for p, (rel, _), n in zip(pred, literals, ncols)]
+    order = sorted(range(len(literals)), key=lambda i: -float(pred[i]))
+    result = [None] * len(literals)
+    exact = np.zeros(len(literals), np.int64)
+    retry = []
+    pos = 0
+    while pos < len(order):  # calls of at most budget_bytes of outputs
+        chunk, used = [], 0
+        while pos < len(order) and (not chunk or used + 4 * caps[order[pos]] * ncols[order[pos]] <= budget_bytes):
+            used += 4 * caps[order[pos]] * ncols[order[pos]]
+            chunk.append(order[pos])
+            pos += 1
+        outs, rows, fits = ctx.create_binding_tables_presized(binding, relations, [literals[i] for i in chunk],
+                                                              [caps[i] for i in chunk])
+        for k, i in enumerate(chunk):
+            exact[i] = int(rows[k])
+            if fits[k]:
+                result[i] = outs[k]
+            else:
+                retry.append(i)
+    for i in retry:  # the exact fallback, one join at a time
+        outs, rows, fits = ctx.create_binding_tables_presized(binding, relations, [literals[i]], [int(exact[i])])
+        assert fits[0], "an exact-size retry must fit"
+        result[i] = outs[0]
+    return result, {"predicted_log2": pred, "capacities": caps, "rows": exact, "fallbacks": len(retry)}

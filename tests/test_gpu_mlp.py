@@ -8,6 +8,7 @@ import pytest
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import mlp_oracle  # noqa: E402
+import oracle  # noqa: E402
 from quickfoil_amd import estimator  # noqa: E402
 
 pytestmark = pytest.mark.gpu
@@ -100,5 +101,46 @@ def test_presized_join_is_exact(qf_ctx, recorded):
         assert fell_back == (want[0].numel() > 1)
         for g, e in zip(got, want):
             assert torch.equal(g, e)
+    finally:
+        est.close()
+
+
+def test_batched_presized_joins_and_forced_fallback(qf_ctx, recorded):
+    """The estimator in the loop: CreateBindingTable for many literals in one call, capacities from
+    the network's predictions, largest prediction first; joins whose rows exceed their capacity
+    write nothing and are rerun at their exact size.  With capacities forced far below the
+    predictions (under-prediction) every non-empty join takes the fallback: the overflow flag is
+    raised, the retry writes, and every output equals the two-pass join and the oracle's rows."""
+    import torch
+    X, y, Xh, yh, w = recorded
+    est = estimator.Estimator(qf_ctx)
+    try:
+        est.fit(X, y, epochs=20, batch=512, lr=3e-3)
+        dev = torch.device("cuda")
+        binding = [torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in w["binding"]]
+        rels = [[torch.from_numpy(np.ascontiguousarray(c)).to(dev) for c in r] for r in w["relations"]]
+        picks = [w["literals"][i] for i in range(0, len(w["literals"]), 53)][:36]
+        for margin in (1.0, -40.0):
+            got, stats = estimator.presized_joins(qf_ctx, est, binding, rels, picks, margin=margin)
+            nonempty = int(sum(1 for r in stats["rows"] if r > 0))
+            if margin < 0:
+                assert stats["fallbacks"] == sum(1 for r in stats["rows"] if r > 16)
+                assert stats["fallbacks"] > 0
+            assert nonempty > 0
+            for (rel, vars_), g, rows in zip(picks, got, stats["rows"]):
+                want = qf_ctx.create_binding_table(binding, rels[rel], vars_)
+                assert want[0].numel() == rows
+                for a, b in zip(g, want):
+                    assert torch.equal(a, b)
+        # Against the oracle's CreateBindingTable as a multiset, on a few literals.
+        sub = picks[:6]
+        prob = oracle.Problem(w["binding"], w["num_positive"], w["relations"], sub)
+        outs, rows, fits = qf_ctx.create_binding_tables_presized(binding, rels, sub, [1 << 22] * len(sub))
+        assert fits.all()
+        for i in range(len(sub)):
+            want = prob.binding_table(i)
+            assert int(rows[i]) == len(want[0])
+            got_rows = sorted(zip(*[c.cpu().numpy().tolist() for c in outs[i]]))
+            assert got_rows == sorted(zip(*[c.tolist() for c in want]))
     finally:
         est.close()
