@@ -1188,8 +1188,7 @@ static std::shared_ptr<HeavyTables> heavy_tables(Context* ctx, RelPart& rp, cons
 // Binding table partitioned on column `var`, carrying `cols` (must include var) and row ids, into
 // the caller's buffers (ncols + 1 columns of rows int32, offsets 2^R + 1 int64).
 static void partition_binding(Context* ctx, const qf_table& b, int var, const std::vector<int>& cols, int R,
-                              const std::vector<int32_t*>& out_cols, int64_t* d_offsets) {
-  PhaseScope ps(ctx, "partition_binding");
+                              const std::vector<int32_t*>& out_cols, int64_t* d_offsets, Scratch* scratch) {
   // Batches of at most 6 data columns, each with the key column, so every batch partitions the
   // same way (the partition is a deterministic, stable function of the key column).
   std::vector<int> others;
@@ -1208,8 +1207,9 @@ static void partition_binding(Context* ctx, const qf_table& b, int var, const st
       out.push_back(out_cols[std::find(cols.begin(), cols.end(), c) - cols.begin()]);
     }
     if (first) out.push_back(out_cols[cols.size()]);
+    if (scratch) scratch->used = 0;  // the previous batch's kernels precede on this stream
     stable_partition(ctx, in.data(), (int)in.size(), 0, first, b.num_rows, 1, R, out.data(), 1,
-                     first ? d_offsets : nullptr);
+                     first ? d_offsets : nullptr, scratch);
     first = false;
   } while (pos < others.size());
 }
@@ -1222,6 +1222,7 @@ struct StepPlan {
   std::vector<int> join_vars;
   std::vector<std::vector<DevBuf>> tparts;  // per join variable: staged columns + row id
   std::vector<DevBuf> toffs;
+  std::vector<DevBuf> tscratch;             // per join variable: its partition's temporaries
   std::vector<std::shared_ptr<RelPart>> keep_rel;
   std::vector<std::shared_ptr<HeavyTables>> keep_heavy;
   std::vector<DevUnit> units;
@@ -1286,10 +1287,15 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.join_vars.assign(vset.begin(), vset.end());
   QF_REQUIRE((int)P.bcols.size() <= EV_MAXTC, "too many binding columns referenced by one step");
   const int64_t rows = std::max<int64_t>(1, binding.num_rows);
+  // The binding partitions of the join variables are independent and run side by side, each on
+  // its own stream with its temporaries in a region of its own (a batch of at most 6 data
+  // columns plus the key and the row id is the widest pass).
+  const int part_cols = (int)std::min<size_t>(P.bcols.size(), 7) + 1;
   for (size_t v = 0; v < P.join_vars.size(); ++v) {
     P.tparts.emplace_back();
     for (size_t c = 0; c <= P.bcols.size(); ++c) P.tparts.back().emplace_back(ctx, rows * 4);
     P.toffs.emplace_back(ctx, (((size_t)1 << P.R_T) + 1) * 8);
+    P.tscratch.emplace_back(ctx, std::max<size_t>(256, partition_scratch_bytes(binding.num_rows, part_cols, P.R_T)));
   }
   auto var_index = [&](int v) { return (int)(std::find(P.join_vars.begin(), P.join_vars.end(), v) - P.join_vars.begin()); };
   auto tidx = [&](int c) { return (int)(std::find(P.bcols.begin(), P.bcols.end(), c) - P.bcols.begin()); };
@@ -1550,16 +1556,57 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.bit_words = ceil_div(binding.num_rows, 64);
 }
 
-// Device work of one step, all on the context stream (capturable: kernels and device copies).
+// Auxiliary streams for forked work: n - 1 streams and n events (event 0 forks, event v joins
+// stream v - 1 back).  Default priority, non-blocking.
+static void ensure_aux_streams(Context* ctx, int n) {
+  while ((int)ctx->aux_streams.size() < n - 1) {
+    hipStream_t st;
+    QF_CHECK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    ctx->aux_streams.push_back(st);
+  }
+  while ((int)ctx->aux_events.size() < n) {
+    hipEvent_t ev;
+    QF_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    ctx->aux_events.push_back(ev);
+  }
+}
+
+// Device work of one step on the context stream, the binding partitions forked to auxiliary
+// streams and joined back (capturable: kernels, device copies and event waits).
 static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_positive, StepPlan& P, uint64_t* d_bits,
                          uint64_t* h_out) {
   fill_u64(ctx, P.d_copies.as<uint64_t>(), 0, (int64_t)P.ncopies * P.num_slots * 4);
   if (d_bits && P.num_slots > 0) fill_u64(ctx, d_bits, 0, (int64_t)P.num_slots * P.bit_words);
   if (!P.units.empty() && binding.num_rows > 0) {
-    for (size_t v = 0; v < P.join_vars.size(); ++v) {
-      std::vector<int32_t*> outs;
-      for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
-      partition_binding(ctx, binding, P.join_vars[v], P.bcols, P.R_T, outs, P.toffs[v].as<int64_t>());
+    {
+      PhaseScope ps(ctx, "partition_binding");
+      const int nv = (int)P.join_vars.size();
+      ensure_aux_streams(ctx, nv);
+      hipStream_t main = ctx->stream;
+      if (nv > 1) QF_CHECK_HIP(hipEventRecord(ctx->aux_events[0], main));
+      for (int v = 0; v < nv; ++v) {
+        // Join variable v on aux stream v - 1 (the first on the context stream), after all work
+        // already queued on the context stream.
+        hipStream_t st = v == 0 ? main : ctx->aux_streams[v - 1];
+        if (v > 0) QF_CHECK_HIP(hipStreamWaitEvent(st, ctx->aux_events[0], 0));
+        std::vector<int32_t*> outs;
+        for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
+        Scratch scratch;
+        scratch.base = P.tscratch[v].as<char>();
+        scratch.capacity = P.tscratch[v].bytes();
+        ctx->stream = st;
+        try {
+          partition_binding(ctx, binding, P.join_vars[v], P.bcols, P.R_T, outs, P.toffs[v].as<int64_t>(), &scratch);
+        } catch (...) {
+          ctx->stream = main;
+          throw;
+        }
+        ctx->stream = main;
+        if (v > 0) {
+          QF_CHECK_HIP(hipEventRecord(ctx->aux_events[v], st));
+          QF_CHECK_HIP(hipStreamWaitEvent(main, ctx->aux_events[v], 0));
+        }
+      }
     }
     EvalArgs a{};
     a.lits = P.d_lits.as<DevLit>();

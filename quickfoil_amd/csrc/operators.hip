@@ -216,6 +216,11 @@ void qf_context_destroy(qf_context* c) {
     (void)hipStreamDestroy(ctx->graph_stream);
   }
   if (ctx->graph_order_event) (void)hipEventDestroy(ctx->graph_order_event);
+  for (hipStream_t st : ctx->aux_streams) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+  for (hipEvent_t ev : ctx->aux_events) (void)hipEventDestroy(ev);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -115,6 +115,10 @@ struct Context {
   // default stream, which cannot be captured); ordered after the caller's stream by an event.
   hipStream_t graph_stream = nullptr;
   hipEvent_t graph_order_event = nullptr;
+  // Streams (default priority) and events for independent work forked from `stream` and joined
+  // back to it (the binding table's partitions per join variable); made on first use.
+  std::vector<hipStream_t> aux_streams;
+  std::vector<hipEvent_t> aux_events;
   // While a step is being captured, released pool blocks move here instead of the pool.
   std::vector<DevBuf>* capture_hold = nullptr;
   bool timing = false;
