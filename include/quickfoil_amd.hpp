@@ -13,8 +13,10 @@
  *                                                 (learner/CandidateLiteralEvaluator.cpp:264-406)
  *
  * Each operator takes a Context first (the reference's operators are free functions over global
- * state).  Tables are device-resident int32 columns viewed by DeviceTableView, which carries the
- * per-column partition and hash-table caches that TableView carries (storage/TableView.hpp:22-88).
+ * state) and, like the reference's, has finished its device work when it returns (it synchronises
+ * the context's stream, which host copies on the default stream do not wait for).  Tables are
+ * device-resident int32 columns viewed by DeviceTableView, which carries the per-column partition
+ * and hash-table caches that TableView carries (storage/TableView.hpp:22-88).
  * Counts are uint64; ToSizeType converts one to the reference's int32 size_type
  * (schema/TypeDefs.hpp:13) and refuses values that would wrap.  Where the reference aborts
  * through glog CHECK, the adapter throws quickfoil_amd::Error (status and qf_last_error text).
@@ -221,6 +223,7 @@ inline void RadixPartition(Context& ctx, int column_id, DeviceTableView* table, 
   ctx.Check(qf_radix_partition(ctx.get(), table->column_at(column_id), n, radix_bits, p->pairs.data(),
                                p->offsets.data()),
             "qf_radix_partition");
+  ctx.Synchronize();
   p->host_offsets = p->offsets.ToHost();
   p->radix_bits = radix_bits;
 }
@@ -261,6 +264,7 @@ inline std::unique_ptr<DeviceFoilHashTable> BuildHashTableOnTable(Context& ctx, 
   t->next = DeviceArray<int32_t>(table.num_tuples());
   ctx.Check(qf_build_hash_table_on_table(ctx.get(), &kt, t->buckets.data(), t->next.data()),
             "qf_build_hash_table_on_table");
+  ctx.Synchronize();
   return t;
 }
 
