@@ -58,6 +58,10 @@ constexpr int EV_PST_WORDS = 1536;     // LDS words for the partition's facts (6
 constexpr int EV_MIN_CAP = 16;
 constexpr int EV_MAXMASK = 64;         // literals per unit whose hit masks prune other literals
 constexpr int EV_MAXBU = 32;           // units per batch (units of equal radix bits share staged rows)
+#ifndef QF_EV_STAGE
+#define QF_EV_STAGE 4
+#endif
+constexpr int EV_STAGE = QF_EV_STAGE;  // binding words a thread stages per round (loads in flight)
 
 struct DevSubset {
   int nk;
@@ -785,10 +789,12 @@ __device__ inline void prefetch_facts(cptr<DevUnit> U, int unit, int64_t q, int6
 }
 
 // 256-thread blocks: a block's barriers wait for four waves only, which measured faster than
-// eight-wave blocks at the same LDS per CU.  Up to six such blocks share a CU when their LDS fits;
-// that takes at most 80 VGPRs per lane (6 waves per SIMD), which the attribute asks for.
+// eight-wave blocks at the same LDS per CU.  At the tiles the autotuner picks for wide steps
+// (768 rows) LDS holds five blocks per CU, so the attribute asks for five waves per SIMD: that
+// leaves up to 96 VGPRs per lane, which the kernel fits without spilling to scratch (at six, 80
+// VGPRs, it spilled and measured 1 % slower).
 #ifndef QF_EV_WAVES_PER_EU
-#define QF_EV_WAVES_PER_EU 6
+#define QF_EV_WAVES_PER_EU 5
 #endif
 // kBits: covered bitvectors were asked for (a.bits); the instance without them compiles their
 // code out of the probe loops.
@@ -866,20 +872,37 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
     }
     __syncthreads();
     const int nseg = uni(sh.nseg);
-    // Stage the binding rows of every segment (same staged columns in every group).
+    // Stage the binding rows of every segment (same staged columns in every group).  Word w of a
+    // segment's ntc + 1 columns of n rows (column-major; the last column is the row id, staged as
+    // a positive flag: a row is positive below num_positive) goes to thread w % EV_THREADS, which
+    // takes EV_STAGE words at a time so that their loads are in flight together.  Lane c of every
+    // wave holds column c's source address; the rounds are block-uniform, so every lane takes
+    // part in the shuffles that hand the addresses out.
     for (int sgi = 0; sgi < nseg; ++sgi) {
       const int g = uni(sh.seg_group[sgi]);
       const int64_t src0 = uni64(sh.seg_src[sgi]);
       const int n = uni(sh.seg_rows[sgi]), dst0 = uni(sh.seg_off[sgi]);
-      for (int c = 0; c < ntc; ++c) {
-        const int32_t* src = B->tcols[g][c] + src0;
-        int32_t* dst = tstage + c * a.tile_rows + dst0;
-        for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i];
-      }
-      {  // row ids become positive flags (a row is positive below num_positive)
-        const int32_t* src = B->tcols[g][ntc] + src0;
-        uint8_t* dst = posf + dst0;
-        for (int i = threadIdx.x; i < n; i += EV_THREADS) dst[i] = src[i] < a.num_positive ? 1 : 0;
+      const uint64_t colp = lane <= ntc ? (uint64_t)(uintptr_t)(B->tcols[g][lane] + src0) : 0;
+      const int words = (ntc + 1) * n;
+      for (int w0 = 0; w0 < words; w0 += EV_STAGE * EV_THREADS) {
+        int32_t v[EV_STAGE];
+        int at[EV_STAGE];
+#pragma unroll
+        for (int j = 0; j < EV_STAGE; ++j) {
+          const int w = w0 + (int)threadIdx.x + j * EV_THREADS;
+          const bool ok = w < words;
+          const int c = ok ? w / n : 0;
+          const int i = ok ? w - c * n : 0;  // words past the end read row 0 of column 0
+          const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)colp, c, 64);
+          const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(colp >> 32), c, 64);
+          v[j] = reinterpret_cast<const int32_t*>((uintptr_t)(((uint64_t)hi << 32) | lo))[i];
+          at[j] = !ok ? -1 : c < ntc ? c * a.tile_rows + dst0 + i : -2 - (dst0 + i);
+        }
+#pragma unroll
+        for (int j = 0; j < EV_STAGE; ++j) {
+          if (at[j] >= 0) tstage[at[j]] = v[j];
+          else if (at[j] <= -2) posf[-2 - at[j]] = v[j] < a.num_positive ? 1 : 0;
+        }
       }
     }
 
