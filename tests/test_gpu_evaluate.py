@@ -288,3 +288,63 @@ def test_split_positive_and_negative_tables(qf_ctx, mode, npos):
     for part, lo, hi in ((bp, 0, npos), (bn, npos, 400)):
         got_rows = np.unpackbits(part.cpu().numpy().view(np.uint8), axis=1, bitorder="little")[:, : hi - lo]
         np.testing.assert_array_equal(got_rows, rows[:, lo:hi])
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_wide_relations_vs_bruteforce(qf_ctx, mode):
+    # Arity 4-6: keys of up to six columns (the kernel's generic key width), fact columns beyond
+    # the three carried in registers between units, and up to 32 key subsets in one unit.
+    binding, npos, rels, lits = random_problem(71, n=400, num_cols=3, rel_shapes=((4, 300), (5, 200), (6, 160)),
+                                               domain=4, max_literals=700)
+    assert max(len(v) for _, v in lits) == 6
+    want, want_bits = oracle.Problem(binding, npos, rels, lits).bruteforce(semi=True)
+    got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, mode=mode, semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)
+
+
+def test_wide_relations_on_hbm_tables(qf_ctx):
+    # One radix bit: every partition holds about half of a 2,000-row relation, more than an LDS
+    # table takes, so every key subset of widths 1-6 is counted in HBM tables.
+    binding, npos, rels, lits = random_problem(72, n=3000, num_cols=3, rel_shapes=((4, 2000), (6, 1800)),
+                                               domain=4, max_literals=500)
+    want, want_bits = oracle.Problem(binding, npos, rels, lits).grouped(semi=True)
+    for mode in MODES:
+        got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, mode=mode, semi_bits=True, radix_bits=1)
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(bits, want_bits)
+
+
+def _plan_counts_bruteforce(binding, npos, rel, keys):
+    """Counts of literals given as key lists [(relation column, binding column), ...] (a relation
+    column may appear twice), by direct comparison of every (binding row, fact) pair."""
+    b = [np.asarray(c, dtype=np.int64) for c in binding]
+    r = [np.asarray(c, dtype=np.int64) for c in rel]
+    out = np.zeros((len(keys), 4), dtype=np.uint64)
+    for i, key in enumerate(keys):
+        m = np.ones((len(b[0]), len(r[0])), dtype=bool)
+        for pc, bc in key:
+            m &= b[bc][:, None] == r[pc][None, :]
+        cnt = m.sum(axis=1)
+        pos = np.arange(len(cnt)) < npos
+        out[i] = [np.sum((cnt > 0) & pos), np.sum((cnt > 0) & ~pos), cnt[pos].sum(), cnt[~pos].sum()]
+    return out
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_plan_with_repeated_relation_columns(qf_ctx, mode):
+    # A plan may bind one relation column to two binding columns (CountAggregator's atoms are
+    # (background column, binding column) pairs, operations/CountAggregator.hpp:25-29); the
+    # library keeps one key position and checks the two binding columns for equality in the row.
+    binding, npos, rels, _ = random_problem(9, n=3000, num_cols=3, rel_shapes=((3, 2000),), domain=6)
+    atoms = [(1, 1), (1, 2), (0, 2)]
+    conj = [(0, 1, 3), (0, 2, 4)]
+    p = {"relation": 0, "join_column": 0, "binding_column": 0, "atoms": atoms, "atom_literals": [0, 1, 2],
+         "conjunctions": conj, "root_literal": 5}
+    keys = [[(0, 0), (1, 1)], [(0, 0), (1, 2)], [(0, 0), (0, 2)], [(0, 0), (1, 1), (1, 2)],
+            [(0, 0), (1, 1), (0, 2)], [(0, 0)]]
+    want = _plan_counts_bruteforce(binding, npos, rels[0], keys)
+    assert want[3, 2] > 0 and want[4, 2] > 0  # the equality literals match some rows
+    got, _ = qf_ctx.count_aggregate([_t(c) for c in binding], npos, [[_t(c) for c in r] for r in rels], [p], 6,
+                                    mode=mode)
+    np.testing.assert_array_equal(got, want)
