@@ -348,3 +348,23 @@ def test_plan_with_repeated_relation_columns(qf_ctx, mode):
     got, _ = qf_ctx.count_aggregate([_t(c) for c in binding], npos, [[_t(c) for c in r] for r in rels], [p], 6,
                                     mode=mode)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain_group"])
+def test_twelve_binding_columns(qf_ctx, mode):
+    # Twelve clause variables: twelve staged columns plus the row id in a work item, up to twelve
+    # join variables in one unit, and binding partitions done in batches of six data columns.
+    binding, npos, rels, lits = random_problem(73, n=2000, num_cols=12, rel_shapes=((2, 900), (3, 700)), domain=7,
+                                               max_literals=400)
+    assert len({v for _, vs in lits for v in vs if v >= 0}) == 12
+    want, want_bits = oracle.Problem(binding, npos, rels, lits).bruteforce(semi=True)
+    got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, mode=mode, semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)
+
+
+def test_thirteen_binding_columns_are_refused(qf_ctx):
+    binding, npos, rels, _ = random_problem(74, n=100, num_cols=13, rel_shapes=((2, 50),), domain=5)
+    lits = [(0, [v, (v + 1) % 13]) for v in range(13)]  # every variable referenced
+    with pytest.raises(Exception, match="too many binding columns"):
+        _gpu_eval(qf_ctx, binding, npos, rels, lits, mode="batched")
