@@ -56,10 +56,12 @@ struct ChainTable {
   std::vector<int32_t> next;
 };
 
-// FoilHashTable(size, radix_bits): P2(size) buckets, mask = (P2(size)-1) << radix_bits.
+// FoilHashTable(size, radix_bits): P2(size) buckets, mask = (P2(size)-1) << radix_bits.  The
+// reference requires size > 0 (storage/FoilHashTable.cpp:36); an empty table (a label side with no
+// rows) gets one empty bucket here, so that probing it finds nothing.
 ChainTable make_table(int64_t size, int radix_bits) {
   ChainTable t;
-  const uint32_t nb = orc_next_pow2((uint32_t)size);
+  const uint32_t nb = size > 0 ? orc_next_pow2((uint32_t)size) : 1u;
   t.mask = (nb - 1u) << radix_bits;
   t.buckets.assign(nb, 0);
   t.next.assign(size, 0);

@@ -194,6 +194,22 @@ def test_label_aware_binding_table(vars_, rel_rows):
     assert prob.label_aware_binding_table(0, int(counts[2]) + 1, int(counts[3]))[0] == -1
 
 
+@pytest.mark.parametrize("npos", [0, 150])
+def test_label_aware_binding_table_with_an_empty_label_side(npos):
+    # No positives (or no negatives) while the relation outgrows both label tables: the relation
+    # probes a table built on an empty side, which must find nothing.
+    binding, _, rels, _ = random_problem(12, n=150, num_cols=2, rel_shapes=((2, 400),), domain=12)
+    prob = oracle.Problem(binding, npos, rels, [(0, [0, 1])])
+    counts = prob.bruteforce()[0][0]
+    assert counts[2] + counts[3] > 0 and min(counts[2], counts[3]) == 0
+    got, cols = prob.label_aware_binding_table(0, int(counts[2]), int(counts[3]))
+    assert got == counts[2] + counts[3]
+    pos, neg = brute_extended_join(binding, npos, rels[0], [0, 1])
+    rows = list(zip(*[c.tolist() for c in cols]))
+    assert sorted(rows[: int(counts[2])]) == pos
+    assert sorted(rows[int(counts[2]):]) == neg
+
+
 @pytest.mark.parametrize("vars_", [[0, 2], [1, 3], [2, 0], [0, 0], [1, 1, 0], [0, 3, 4], [-1, 1], [1, -1, 2]])
 @pytest.mark.parametrize("rel_rows", [30, 400])
 def test_create_binding_table_equals_bruteforce(vars_, rel_rows):
