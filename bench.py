@@ -38,11 +38,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_probe_bytes(workload):
+def single_key_units(literals):
+    """(relation, join position) units all of whose literals bind one position: the library's
+    batched form (without covered bitvectors) counts them from value runs in k_scalar_units, and
+    the other units in k_eval.  The join position is the first bound one (no argument types)."""
+    single = {}
+    for rel, vars_ in literals:
+        bound = [i for i, v in enumerate(vars_) if v >= 0]
+        key = (rel, bound[0])
+        single[key] = single.get(key, True) and len(bound) == 1
+    return {k for k, one in single.items() if one}
+
+
+def algorithmic_probe_bytes(workload, units=None, exclude=False):
     """Per step: for each join group (relation, join column, clause variable), the relation's
     rows (8 B: join value + row id), the binding rows (8 B), 4 B per relation atom column and per
     binding atom column, and 32 B of counts per literal (the per-group formula used for the
-    roofline; per-match random gathers are not counted)."""
+    roofline; per-match random gathers are not counted).  `units`: only the groups of these
+    (relation, join position) units, or of all the others with `exclude`."""
     from quickfoil_amd import datagen  # noqa: F401
     rels = workload["relations"]
     nb = int(workload["binding"][0].numel())
@@ -50,6 +63,8 @@ def algorithmic_probe_bytes(workload):
     for rel, vars_ in workload["literals"]:
         bound = [i for i, v in enumerate(vars_) if v >= 0]
         jk = bound[0]
+        if units is not None and ((rel, jk) in units) == exclude:
+            continue
         key = (rel, jk, vars_[jk])
         g = groups.setdefault(key, {"bg": set(), "bind": set(), "lits": 0})
         g["lits"] += 1
@@ -373,12 +388,19 @@ def main():
     copy_gbs = copy_ceiling(torch, dev)
     log("device copy ceiling: %.0f GB/s" % copy_gbs)
     algo_bytes, num_groups = algorithmic_probe_bytes(w)
+    # k_eval takes every unit but the single-key ones, which k_scalar_units counts; each kernel's
+    # roofline is its own groups' bytes over its own phase time.
+    single = single_key_units(w["literals"])
+    eval_bytes, eval_groups = algorithmic_probe_bytes(w, single, exclude=True)
+    sk_bytes, sk_groups = algorithmic_probe_bytes(w, single)
     part_bytes = algorithmic_partition_bytes(w)
     part_ms_per_step = phases.get("partition_binding", (0.0, 0))[0] / max(1, args.steps)
     part_achieved = part_bytes / (part_ms_per_step * 1e-3) / 1e9 if part_ms_per_step > 0 else 0.0
     probe_ms, probe_launches = phases.get("probe_count", (0.0, 0))
     probe_ms_per_step = probe_ms / max(1, args.steps)
-    achieved = algo_bytes / (probe_ms_per_step * 1e-3) / 1e9 if probe_ms_per_step > 0 else 0.0
+    achieved = eval_bytes / (probe_ms_per_step * 1e-3) / 1e9 if probe_ms_per_step > 0 else 0.0
+    sk_ms_per_step = phases.get("probe_count_single_key", (0.0, 0))[0] / max(1, args.steps)
+    sk_achieved = sk_bytes / (sk_ms_per_step * 1e-3) / 1e9 if sk_ms_per_step > 0 else 0.0
 
     cpu = None
     if not args.no_cpu:
@@ -418,8 +440,15 @@ def main():
         "phase_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in phases.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
-                     "kernel": "k_eval (probe and count; HIP events around each launch on the context stream)",
-                     "algorithmic_bytes_per_step": algo_bytes},
+                     "kernel": "k_eval (probe and count of %d of the %d join groups; HIP events around each "
+                               "launch on the context stream)" % (eval_groups, num_groups),
+                     "algorithmic_bytes_per_step": eval_bytes},
+        "roofline_single_key": {"bound": "hbm", "achieved": sk_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": sk_achieved / HBM_PEAK_GBS, "traffic": None,
+                                "kernel": "k_scalar_units (the %d join groups of units keyed on one column; HIP "
+                                          "events around each launch)" % sk_groups,
+                                "algorithmic_bytes_per_step": sk_bytes, "ms_per_step": sk_ms_per_step},
+        "probe_phase_ms_per_step": probe_ms_per_step + sk_ms_per_step,
         "roofline_partition": {"bound": "hbm", "achieved": part_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": part_achieved / HBM_PEAK_GBS, "traffic": None,
                                "kernel": "binding radix partition per join variable (histogram, scan, scatter, "

@@ -332,6 +332,230 @@ __global__ void k_count_items(const DevBatch* batches, const DevUnit* units, con
   counts[cell] = (rows + tile - 1) / tile;
 }
 
+// ------------------------------------------------------------------------------------------
+// Single-key units: a (relation, join column) all of whose literals key on the join column alone
+// (in the step, a relation's last column).  Their counts need only each binding row's number of
+// facts with its join value, cnt(r) = #{s : s[c] = r[v]}, so per value x of a cell:
+//   binding_pos += pos(x) * cnt(x),   covered_pos += pos(x) if cnt(x) > 0,
+// with pos(x) the positive binding rows with value x (likewise negatives).  The relation side is
+// reduced once to runs (value, count) per partition; a work item (cell, chunk of at most SC_ROWS
+// rows of each join variable) counts its rows' values in LDS tables, one per join variable, and
+// streams every such unit's runs of the cell through them.  A run is a distinct value, so each
+// table entry a unit's runs hit adds its rows to the covered counts once.
+
+// Run heads of a column sorted by mix32 (equal values adjacent): flags[i] = 1 at a new value.
+__global__ void k_run_heads(const int32_t* v, int64_t n, int64_t* flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = (i == 0 || v[i] != v[i - 1]) ? 1 : 0;
+}
+
+// Runs from the scanned heads (starts[i] = heads before row i): each run's value and first row,
+// and per partition its first run (partition p starts at row poff[p]; a run never crosses a
+// partition because equal values share their partition).  Threads cover max(n, parts + 1).
+__global__ void k_run_compact(const int32_t* v, int64_t n, const int64_t* flags, const int64_t* starts,
+                              int32_t* run_val, int32_t* run_row, const int64_t* poff, int64_t parts,
+                              int64_t* run_off) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flags[i]) {
+    run_val[starts[i]] = v[i];
+    run_row[starts[i]] = (int32_t)i;
+  }
+  if (i <= parts) {
+    const int64_t row = poff[i];
+    run_off[i] = row < n ? starts[row] : (n > 0 ? starts[n - 1] + flags[n - 1] : 0);
+  }
+}
+
+// Run lengths from consecutive first rows; the last run ends at row n.
+__global__ void k_run_lengths(const int32_t* run_row, int64_t runs, int64_t n, int32_t* run_cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < runs) run_cnt[r] = (int32_t)((r + 1 < runs ? (int64_t)run_row[r + 1] : n) - run_row[r]);
+}
+
+constexpr int SC_THREADS = 256;
+constexpr int SC_VARS = 4;     // join variables whose tables one pass holds
+constexpr int SC_ROWS = 256;   // binding rows of each join variable per work item
+constexpr int SC_CAP = 512;    // table slots per join variable (load at most 1/2)
+constexpr int SC_UNITS = 32;   // single-key units per launch (bits of a flag word)
+
+struct ScalarUnit {
+  const int32_t* run_val;  // the relation's runs on the join column, partition by partition
+  const int32_t* run_cnt;
+  const int64_t* run_off;  // per cell: first run
+  int lit[EV_MAXG][2];     // per join variable of the step: its literals [lit[v][0], lit[v][1])
+};
+
+struct ScalarArgs {
+  const ScalarUnit* units;
+  int nunits;
+  int nvars;
+  const int32_t* vals[EV_MAXG];  // join variable v's values in its partition's row order
+  const int32_t* tids[EV_MAXG];  // and the rows' ids
+  const int64_t* toff[EV_MAXG];  // cell starts of that partition
+  int64_t cells;
+  int num_positive;
+  const DevLit* lits;
+  const DevItem* items;
+  const int64_t* num_items;
+  unsigned long long* counts_out;  // ncopies x num_slots x 4
+  int64_t copy_words;
+  int ncopies;
+};
+
+// Items per cell: the chunks of its longest join variable's rows, none when no unit has a run
+// there.
+__global__ void k_scalar_count(ScalarArgs a, int64_t* counts) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.cells) return;
+  bool runs = false;
+  for (int u = 0; u < a.nunits; ++u) runs = runs || a.units[u].run_off[q + 1] > a.units[u].run_off[q];
+  int64_t rows = 0;
+  if (runs)
+    for (int v = 0; v < a.nvars; ++v) rows = max(rows, a.toff[v][q + 1] - a.toff[v][q]);
+  counts[q] = (rows + SC_ROWS - 1) / SC_ROWS;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scalar_units(ScalarArgs a) {
+  __shared__ uint32_t word[SC_VARS][SC_CAP];  // 0 empty, else fingerprint | (1 + staged row)
+  __shared__ uint32_t pn[SC_VARS][SC_CAP];    // the value's positive rows << 16 | negative rows
+  __shared__ uint32_t hit[SC_VARS][SC_CAP];   // bit u: a run of unit u has the value
+  __shared__ int32_t sval[SC_VARS][SC_ROWS];
+  __shared__ uint8_t spos[SC_VARS][SC_ROWS];
+  __shared__ unsigned long long acc[SC_UNITS][SC_VARS][4];
+  __shared__ int64_t ubeg[SC_UNITS];      // per unit: its first run in the item's cell
+  __shared__ int ustart[SC_UNITS + 1];    // per unit: its first position among the cell's runs
+  __shared__ uint32_t uvars[SC_UNITS];    // per unit: bit j set when it has literals on variable j
+  static_assert(SC_ROWS < (1 << 16) && SC_ROWS <= (int)EV_FIELD_MASK, "row counts and rows fit their fields");
+  static_assert(SC_UNITS <= 64, "one lane per unit");
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t total = uni64(*a.num_items);
+  cptr<ScalarUnit> units = as_const(a.units);
+  unsigned long long* copy = a.counts_out + (int64_t)(blockIdx.x % a.ncopies) * a.copy_words;
+  // The tables start empty; the pass over them that ends every item empties them again.
+  for (int i = threadIdx.x; i < SC_VARS * SC_CAP; i += SC_THREADS) {
+    (&word[0][0])[i] = 0;
+    (&pn[0][0])[i] = 0;
+    (&hit[0][0])[i] = 0;
+  }
+  for (int vg = 0; vg < a.nvars; vg += SC_VARS) {
+    const int nv = min(SC_VARS, a.nvars - vg);
+    for (int i = threadIdx.x; i < SC_UNITS * SC_VARS * 4; i += SC_THREADS) (&acc[0][0][0])[i] = 0;
+    for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+      const int64_t q = a.items[it].q;
+      const int64_t r0 = (int64_t)a.items[it].chunk * SC_ROWS;
+      __syncthreads();  // the previous item is done with the staged rows and the unit map
+      int n[SC_VARS];
+#pragma unroll
+      for (int j = 0; j < SC_VARS; ++j) {
+        n[j] = 0;
+        if (j < nv) {
+          const int64_t t0 = a.toff[vg + j][q], t1 = a.toff[vg + j][q + 1];
+          n[j] = (int)max<int64_t>(0, min<int64_t>(SC_ROWS, t1 - t0 - r0));
+          for (int r = threadIdx.x; r < n[j]; r += SC_THREADS) {
+            sval[j][r] = a.vals[vg + j][t0 + r0 + r];
+            spos[j][r] = a.tids[vg + j][t0 + r0 + r] < a.num_positive ? 1 : 0;
+          }
+        }
+      }
+      if (wave == 0) {  // the cell's runs, unit by unit (one lane per unit)
+        int cnt = 0;
+        uint32_t vars = 0;
+        if (lane < a.nunits) {
+          cptr<ScalarUnit> U = units + lane;
+          const int64_t b0 = U->run_off[q];
+          cnt = (int)(U->run_off[q + 1] - b0);
+          ubeg[lane] = b0;
+          for (int j = 0; j < nv; ++j)
+            if (n[j] > 0 && U->lit[vg + j][1] > U->lit[vg + j][0]) vars |= 1u << j;
+          uvars[lane] = vars;
+          if (vars == 0) cnt = 0;
+        }
+        int x = cnt;  // inclusive prefix over the wave
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(x, off, 64);
+          if (lane >= off) x += y;
+        }
+        if (lane < a.nunits) ustart[lane] = x - cnt;
+        if (lane == a.nunits - 1) ustart[a.nunits] = x;
+      }
+      __syncthreads();  // rows staged, unit map ready (tables empty)
+      for (int i = threadIdx.x; i < SC_VARS * SC_ROWS; i += SC_THREADS) {
+        const int j = i / SC_ROWS, r = i - j * SC_ROWS;
+        if (j >= nv || r >= n[j]) continue;
+        const int32_t x = sval[j][r];
+        const uint32_t h = lds_hash(&x, 1), fp = lds_fp(h);
+        uint32_t slot = lds_slot(h, SC_CAP);
+        while (true) {
+          const uint32_t prev = atomicCAS(&word[j][slot], 0u, fp | ((uint32_t)(r + 1) << EV_REP_SHIFT));
+          if (prev == 0) break;
+          if ((prev & EV_FP_MASK) == fp && sval[j][((prev >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1] == x) break;
+          slot = (slot + 1) & (SC_CAP - 1);
+        }
+        atomicAdd(&pn[j][slot], spos[j][r] ? (1u << 16) : 1u);
+      }
+      __syncthreads();  // tables ready
+      // Every (unit, run) of the cell, one per thread at a time.
+      const int nruns = ustart[a.nunits];
+      for (int i = threadIdx.x; i < nruns; i += SC_THREADS) {
+        int u = 0;
+        for (int step = SC_UNITS / 2; step > 0; step >>= 1)
+          if (u + step < a.nunits && ustart[u + step] <= i) u += step;
+        const int64_t run = ubeg[u] + (i - ustart[u]);
+        const uint32_t vars = uvars[u];
+        const int32_t x = units[u].run_val[run];
+        const uint64_t c = (uint32_t)units[u].run_cnt[run];
+        const uint32_t h = lds_hash(&x, 1), fp = lds_fp(h);
+#pragma unroll
+        for (int j = 0; j < SC_VARS; ++j) {
+          if (!((vars >> j) & 1u)) continue;
+          uint32_t slot = lds_slot(h, SC_CAP);
+          while (true) {
+            const uint32_t w = word[j][slot];
+            if (w == 0) break;
+            if ((w & EV_FP_MASK) == fp && sval[j][((w >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1] == x) {
+              const uint32_t rows = pn[j][slot];
+              if (rows >> 16) atomicAdd(&acc[u][j][2], (unsigned long long)(c * (rows >> 16)));
+              if (rows & 0xffffu) atomicAdd(&acc[u][j][3], (unsigned long long)(c * (rows & 0xffffu)));
+              atomicOr(&hit[j][slot], 1u << u);
+              break;
+            }
+            slot = (slot + 1) & (SC_CAP - 1);
+          }
+        }
+      }
+      __syncthreads();  // every unit's hits recorded
+      // Covered rows: each value a unit's runs hit, once; the slots are emptied for the next item.
+      for (int i = threadIdx.x; i < SC_VARS * SC_CAP; i += SC_THREADS) {
+        if ((&word[0][0])[i] == 0) continue;
+        uint32_t f = (&hit[0][0])[i];
+        const uint32_t rows = (&pn[0][0])[i];
+        (&word[0][0])[i] = 0;
+        (&pn[0][0])[i] = 0;
+        (&hit[0][0])[i] = 0;
+        const int j = i / SC_CAP;
+        while (f) {
+          const int u = __ffs((int)f) - 1;
+          f &= f - 1;
+          if (rows >> 16) atomicAdd(&acc[u][j][0], (unsigned long long)(rows >> 16));
+          if (rows & 0xffffu) atomicAdd(&acc[u][j][1], (unsigned long long)(rows & 0xffffu));
+        }
+      }
+    }
+    __syncthreads();  // this pass's counts complete
+    cptr<DevLit> lits = as_const(a.lits);
+    for (int i = threadIdx.x; i < a.nunits * nv * 4; i += SC_THREADS) {
+      const int u = i / (nv * 4), j = (i / 4) % nv, c = i % 4;
+      const unsigned long long v = acc[u][j][c];
+      if (v == 0) continue;
+      for (int li = units[u].lit[vg + j][0]; li < units[u].lit[vg + j][1]; ++li)
+        atomicAdd(copy + (int64_t)lits[li].slot * 4 + c, v);
+    }
+    __syncthreads();  // before the next pass clears acc
+  }
+}
+
 // One thread per work item (the grid covers an upper bound of the item count): item i belongs to
 // the last cell whose scanned start is <= i (cells without items share their start with the
 // next cell).  Thread 0 writes the total.  A cell with many items (a frequent join value) is
@@ -1101,12 +1325,20 @@ struct HeavyTables {
                 // table, so that nothing waits for the build kernels)
 };
 
+// A relation's join column as runs of equal values, partition by partition (single-key units).
+struct Runs {
+  DevBuf val, cnt;  // int32 per run
+  DevBuf off;       // int64 per partition + 1: first run
+};
+
 // A relation partitioned on one join column, with its heavy partitions and their tables.
 struct RelPart {
   std::shared_ptr<PartitionedTable> pt;
+  int join_col = 0;
   DevBuf hq;                     // int32 per partition
   std::vector<int64_t> heavy_q;  // heavy partitions
   std::map<std::vector<int>, std::shared_ptr<HeavyTables>> tables;
+  std::shared_ptr<Runs> runs;
 };
 
 struct RelCache {
@@ -1143,6 +1375,7 @@ static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table&
   pt->h_offsets.resize(((size_t)1 << R) + 1);
   copy_d2h(ctx, pt->h_offsets.data(), pt->offsets.get(), pt->h_offsets.size() * 8);
   rp->pt = pt;
+  rp->join_col = join_col;
   // Heavy: the partition's facts or its LDS table do not fit.
   const int64_t parts = (int64_t)1 << R;
   std::vector<int32_t> hq(parts, -1);
@@ -1157,6 +1390,48 @@ static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table&
   copy_h2d(ctx, rp->hq.get(), hq.data(), parts * 4);
   if (cache) rc.parts[key] = rp;
   return rp;
+}
+
+// The partitioned join column sorted by all 32 bits of mix32 (its top radix bits are the
+// partition, so every partition keeps its rows) and cut into runs of equal values; made once per
+// relation partition and cached with it.
+static std::shared_ptr<Runs> relation_runs(Context* ctx, RelPart& rp) {
+  if (rp.runs) return rp.runs;
+  PhaseScope ps(ctx, "heavy_tables");
+  const int64_t n = rp.pt->num_rows;
+  const int64_t parts = (int64_t)1 << rp.pt->radix_bits;
+  auto ru = std::make_shared<Runs>();
+  DevBuf sorted(ctx, std::max<int64_t>(1, n) * 4), flags(ctx, std::max<int64_t>(1, n) * 8),
+      starts(ctx, std::max<int64_t>(1, n) * 8);
+  const int32_t* in = rp.pt->columns[rp.join_col].as<int32_t>();
+  int32_t* out = sorted.as<int32_t>();
+  stable_partition(ctx, &in, 1, 0, false, n, 1, 32, &out, 1, nullptr);
+  int64_t runs = 0;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_run_heads, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, out, n,
+                       flags.as<int64_t>());
+    QF_CHECK_HIP(hipGetLastError());
+    scan_exclusive_i64(ctx, flags.as<int64_t>(), starts.as<int64_t>(), n);
+    int64_t last[2];
+    copy_d2h(ctx, &last[0], starts.as<int64_t>() + n - 1, 8);
+    copy_d2h(ctx, &last[1], flags.as<int64_t>() + n - 1, 8);
+    runs = last[0] + last[1];
+  }
+  ru->val = DevBuf(ctx, std::max<int64_t>(1, runs) * 4);
+  ru->cnt = DevBuf(ctx, std::max<int64_t>(1, runs) * 4);
+  ru->off = DevBuf(ctx, (parts + 1) * 8);
+  DevBuf row(ctx, std::max<int64_t>(1, runs) * 4);
+  hipLaunchKernelGGL(k_run_compact, dim3((unsigned)ceil_div(std::max(n, parts + 1), 256)), dim3(256), 0, ctx->stream,
+                     out, n, flags.as<int64_t>(), starts.as<int64_t>(), ru->val.as<int32_t>(), row.as<int32_t>(),
+                     rp.pt->offsets.as<int64_t>(), parts, ru->off.as<int64_t>());
+  QF_CHECK_HIP(hipGetLastError());
+  if (runs > 0) {
+    hipLaunchKernelGGL(k_run_lengths, dim3((unsigned)ceil_div(runs, 256)), dim3(256), 0, ctx->stream,
+                       row.as<int32_t>(), runs, n, ru->cnt.as<int32_t>());
+    QF_CHECK_HIP(hipGetLastError());
+  }
+  rp.runs = ru;
+  return ru;
 }
 
 static std::shared_ptr<HeavyTables> heavy_tables(Context* ctx, RelPart& rp, const std::vector<int>& pcols) {
@@ -1249,6 +1524,11 @@ struct StepPlan {
   std::vector<std::shared_ptr<RelPart>> keep_rel;
   std::vector<std::shared_ptr<HeavyTables>> keep_heavy;
   std::vector<DevUnit> units;
+  std::vector<char> unit_scalar;            // per unit: counted by k_scalar_units instead of k_eval
+  std::vector<ScalarUnit> sunits;
+  std::vector<std::shared_ptr<Runs>> keep_runs;
+  DevBuf d_sunits, d_sc_cells, d_sc_counts, d_sc_starts, d_sc_items, d_sc_num;
+  int64_t sc_items = 0;                     // upper bound of the single-key work items
   std::vector<DevBatch> batches;
   std::vector<DevLit> lits;
   std::vector<std::pair<int, int>> launches;  // unit ranges launched together
@@ -1275,8 +1555,11 @@ struct StepPlan {
 
 // Units: (relation, join column) with a group per join variable (batched), one group per
 // literal-group (plain group), or one literal each (plain literal).
+// scalar_units: units whose literals all key on the join column alone are counted from value
+// runs by k_scalar_units (the batched form without covered bitvectors).
 static void plan_step(Context* ctx, const qf_table& binding, const qf_table* relations,
-                      const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P) {
+                      const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P,
+                      bool scalar_units) {
   const bool cache = o.reuse_partitions != 0;
   P.num_slots = num_slots;
   P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
@@ -1457,10 +1740,26 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       else du.child_subsets[dl.pmask] |= 1u << dl.subset;
     }
     du.nsub = (int)subsets.size();
+    const bool scalar = scalar_units && du.nsub == 1;
+    P.unit_scalar.push_back(scalar ? 1 : 0);
+    if (scalar) {
+      auto ru = relation_runs(ctx, rp);
+      P.keep_runs.push_back(ru);
+      ScalarUnit su{};
+      su.run_val = ru->val.as<int32_t>();
+      su.run_cnt = ru->cnt.as<int32_t>();
+      su.run_off = ru->off.as<int64_t>();
+      for (int gi = 0; gi < du.ngroups; ++gi) {
+        const int vi = var_index(S.groups[gi]->join_var);
+        su.lit[vi][0] = du.g[gi].sbeg[0];
+        su.lit[vi][1] = du.g[gi].sbeg[1];
+      }
+      P.sunits.push_back(su);
+    }
     for (int s = 0; s < du.nsub; ++s) {
       du.sub[s].nk = (int)subsets[s].first.size();
       for (int k = 0; k < du.sub[s].nk; ++k) du.sub[s].pcol[k] = subsets[s].first[k];
-      if (!rp.heavy_q.empty()) {
+      if (!rp.heavy_q.empty() && !scalar) {
         auto ht = heavy_tables(ctx, rp, subsets[s].first);
         du.sub[s].hbuf = ht->buf.as<uint32_t>();
         du.sub[s].hbase = ht->base.as<int64_t>();
@@ -1478,7 +1777,8 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   std::vector<std::vector<int>> members;
   if (o.mode == QF_EVAL_BATCHED) {
     std::map<int, std::vector<int>> by_r;
-    for (int u = 0; u < (int)P.units.size(); ++u) by_r[P.units[u].R].push_back(u);
+    for (int u = 0; u < (int)P.units.size(); ++u)
+      if (!P.unit_scalar[u]) by_r[P.units[u].R].push_back(u);
     for (auto& kv : by_r) {
       std::vector<int> cur;
       std::set<int> cur_vars;
@@ -1544,6 +1844,21 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     P.max_cells = std::max(P.max_cells, c);
     P.max_items = std::max(P.max_items, items);
     P.launch_items.push_back(items);
+  }
+  if (!P.sunits.empty()) {
+    const int64_t cells = (int64_t)1 << P.R_T;
+    P.d_sunits = DevBuf(ctx, P.sunits.size() * sizeof(ScalarUnit));
+    copy_h2d(ctx, P.d_sunits.get(), P.sunits.data(), P.sunits.size() * sizeof(ScalarUnit));
+    const int64_t cell_range[2] = {0, cells};
+    P.d_sc_cells = DevBuf(ctx, sizeof(cell_range));
+    copy_h2d(ctx, P.d_sc_cells.get(), cell_range, sizeof(cell_range));
+    P.d_sc_counts = DevBuf(ctx, cells * 8);
+    P.d_sc_starts = DevBuf(ctx, cells * 8);
+    // A cell's items are the chunks of its longest join variable: at most one more than the
+    // chunks of all its join variables' rows together.
+    P.sc_items = cells + ceil_div((int64_t)P.join_vars.size() * binding.num_rows, SC_ROWS) + 1;
+    P.d_sc_items = DevBuf(ctx, P.sc_items * sizeof(DevItem));
+    P.d_sc_num = DevBuf(ctx, 8);
   }
   P.d_units = DevBuf(ctx, std::max<size_t>(1, P.units.size()) * sizeof(DevUnit));
   P.d_batches = DevBuf(ctx, std::max<size_t>(1, P.batches.size()) * sizeof(DevBatch));
@@ -1630,6 +1945,42 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
           QF_CHECK_HIP(hipStreamWaitEvent(main, ctx->aux_events[v], 0));
         }
       }
+    }
+    // Single-key units, 32 at a time (one flag bit each).
+    for (size_t u0 = 0; u0 < P.sunits.size(); u0 += SC_UNITS) {
+      ScalarArgs sa{};
+      sa.units = P.d_sunits.as<ScalarUnit>() + u0;
+      sa.nunits = (int)std::min<size_t>(SC_UNITS, P.sunits.size() - u0);
+      sa.nvars = (int)P.join_vars.size();
+      for (int v = 0; v < sa.nvars; ++v) {
+        const int c = (int)(std::find(P.bcols.begin(), P.bcols.end(), P.join_vars[v]) - P.bcols.begin());
+        sa.vals[v] = P.tparts[v][c].as<int32_t>();
+        sa.tids[v] = P.tparts[v][P.bcols.size()].as<int32_t>();
+        sa.toff[v] = P.toffs[v].as<int64_t>();
+      }
+      sa.cells = (int64_t)1 << P.R_T;
+      sa.num_positive = (int)num_positive;
+      sa.lits = P.d_lits.as<DevLit>();
+      sa.items = P.d_sc_items.as<DevItem>();
+      sa.num_items = P.d_sc_num.as<int64_t>();
+      sa.counts_out = reinterpret_cast<unsigned long long*>(P.d_copies.get());
+      sa.copy_words = (int64_t)P.num_slots * 4;
+      sa.ncopies = P.ncopies;
+      {
+        PhaseScope ps(ctx, "work_items");
+        hipLaunchKernelGGL(k_scalar_count, dim3((unsigned)ceil_div(sa.cells, 256)), dim3(256), 0, ctx->stream, sa,
+                           P.d_sc_counts.as<int64_t>());
+        QF_CHECK_HIP(hipGetLastError());
+        scan_exclusive_i64(ctx, P.d_sc_counts.as<int64_t>(), P.d_sc_starts.as<int64_t>(), sa.cells);
+        hipLaunchKernelGGL(k_expand_items, dim3((unsigned)ceil_div(P.sc_items, 256)), dim3(256), 0, ctx->stream,
+                           P.d_sc_cells.as<int64_t>(), 1, P.d_sc_counts.as<int64_t>(), P.d_sc_starts.as<int64_t>(),
+                           P.d_sc_items.as<DevItem>(), P.d_sc_num.as<int64_t>());
+        QF_CHECK_HIP(hipGetLastError());
+      }
+      PhaseScope ps(ctx, "probe_count_single_key");
+      const int blocks = (int)std::min<int64_t>((int64_t)ctx->num_cus * 4, P.sc_items);
+      hipLaunchKernelGGL(k_scalar_units, dim3((unsigned)blocks), dim3(SC_THREADS), 0, ctx->stream, sa);
+      QF_CHECK_HIP(hipGetLastError());
     }
     EvalArgs a{};
     a.lits = P.d_lits.as<DevLit>();
@@ -1809,7 +2160,7 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
     }
   }
   std::unique_ptr<StepPlan> P(new StepPlan);
-  plan_step(ctx, binding, relations, specs, num_slots, o, *P);
+  plan_step(ctx, binding, relations, specs, num_slots, o, *P, o.mode == QF_EVAL_BATCHED && d_bits == nullptr);
   QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->pinned), (size_t)num_slots * 4 * sizeof(uint64_t),
                              hipHostMallocDefault));
   if (!key.empty()) {

@@ -368,3 +368,32 @@ def test_thirteen_binding_columns_are_refused(qf_ctx):
     lits = [(0, [v, (v + 1) % 13]) for v in range(13)]  # every variable referenced
     with pytest.raises(Exception, match="too many binding columns"):
         _gpu_eval(qf_ctx, binding, npos, rels, lits, mode="batched")
+
+
+@pytest.mark.parametrize("zipf", [False, True])
+def test_single_key_units(qf_ctx, zipf):
+    # Forty relations of arity 1 and one of arity 2 keyed on its second column: 41 units whose
+    # literals key on the join column alone, counted from value runs in the batched form (two
+    # launches of at most 32 units), over six join variables (two passes of four), with skewed
+    # binding values (cells of many 256-row chunks) and relations holding one value many times.
+    rng = np.random.Generator(np.random.PCG64(81 + zipf))
+    n, domain = 30000, 300
+
+    def col(size, s=1.3):
+        if zipf:
+            return (rng.zipf(s, size=size) % domain).astype(np.int32)
+        return rng.integers(0, domain, size=size, dtype=np.int64).astype(np.int32)
+
+    binding = [col(n) for _ in range(6)]
+    rels = [[col(int(rng.choice([1, 20, 900, 20000])))] for _ in range(40)]
+    rels.append([col(5000), col(5000)])
+    lits = [(r, [v]) for r in range(40) for v in range(6) if rng.random() < 0.7]
+    lits += [(40, [-1, v]) for v in range(6)] + [(3, [2])]  # a repeated literal shares its counts
+    npos = 11000
+    want, want_bits = oracle.Problem(binding, npos, rels, lits).grouped(semi=True)
+    got, _ = _gpu_eval(qf_ctx, binding, npos, rels, lits, mode="batched")
+    np.testing.assert_array_equal(got, want)
+    # With covered bitvectors the same units go through the general kernel.
+    got, bits = _gpu_eval(qf_ctx, binding, npos, rels, lits, mode="batched", semi_bits=True)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(bits, want_bits)
