@@ -38,33 +38,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def single_key_units(literals):
-    """(relation, join position) units all of whose literals bind one position: the library's
-    batched form (without covered bitvectors) counts them from value runs in k_scalar_units, and
-    the other units in k_eval.  The join position is the first bound one (no argument types)."""
-    single = {}
-    for rel, vars_ in literals:
-        bound = [i for i, v in enumerate(vars_) if v >= 0]
-        key = (rel, bound[0])
-        single[key] = single.get(key, True) and len(bound) == 1
-    return {k for k, one in single.items() if one}
-
-
-def algorithmic_probe_bytes(workload, units=None, exclude=False):
+def algorithmic_probe_bytes(workload):
     """Per step: for each join group (relation, join column, clause variable), the relation's
     rows (8 B: join value + row id), the binding rows (8 B), 4 B per relation atom column and per
     binding atom column, and 32 B of counts per literal (the per-group formula used for the
-    roofline; per-match random gathers are not counted).  `units`: only the groups of these
-    (relation, join position) units, or of all the others with `exclude`."""
-    from quickfoil_amd import datagen  # noqa: F401
+    roofline; per-match random gathers are not counted)."""
     rels = workload["relations"]
     nb = int(workload["binding"][0].numel())
     groups = {}
     for rel, vars_ in workload["literals"]:
         bound = [i for i, v in enumerate(vars_) if v >= 0]
         jk = bound[0]
-        if units is not None and ((rel, jk) in units) == exclude:
-            continue
         key = (rel, jk, vars_[jk])
         g = groups.setdefault(key, {"bg": set(), "bind": set(), "lits": 0})
         g["lits"] += 1
@@ -91,10 +75,10 @@ def algorithmic_partition_bytes(workload):
     return len(join_vars) * nb * 4 * (2 * len(cols) + 1)
 
 
-def measured_traffic(workload):
-    """HBM bytes per k_eval launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
-    this workload (profiles/k_eval_traffic.json), or None when none was recorded."""
-    path = os.path.join(ROOT, "profiles", "k_eval_traffic.json")
+def measured_traffic(workload, kernel="k_stream"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    of this workload (profiles/<kernel>_traffic.json), or None when none was recorded."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.json" % kernel)
     try:
         with open(path) as f:
             rec = json.load(f).get(workload)
@@ -388,19 +372,16 @@ def main():
     copy_gbs = copy_ceiling(torch, dev)
     log("device copy ceiling: %.0f GB/s" % copy_gbs)
     algo_bytes, num_groups = algorithmic_probe_bytes(w)
-    # k_eval takes every unit but the single-key ones, which k_scalar_units counts; each kernel's
-    # roofline is its own groups' bytes over its own phase time.
-    single = single_key_units(w["literals"])
-    eval_bytes, eval_groups = algorithmic_probe_bytes(w, single, exclude=True)
-    sk_bytes, sk_groups = algorithmic_probe_bytes(w, single)
     part_bytes = algorithmic_partition_bytes(w)
     part_ms_per_step = phases.get("partition_binding", (0.0, 0))[0] / max(1, args.steps)
     part_achieved = part_bytes / (part_ms_per_step * 1e-3) / 1e9 if part_ms_per_step > 0 else 0.0
-    probe_ms, probe_launches = phases.get("probe_count", (0.0, 0))
-    probe_ms_per_step = probe_ms / max(1, args.steps)
-    achieved = eval_bytes / (probe_ms_per_step * 1e-3) / 1e9 if probe_ms_per_step > 0 else 0.0
-    sk_ms_per_step = phases.get("probe_count_single_key", (0.0, 0))[0] / max(1, args.steps)
-    sk_achieved = sk_bytes / (sk_ms_per_step * 1e-3) / 1e9 if sk_ms_per_step > 0 else 0.0
+    # The probe phase: k_stream counts every unit in the cells whose binding rows fit its work
+    # items, k_eval the rest (cells of heavily repeated join values); the roofline takes the
+    # step's algorithmic bytes over both kernels' time.
+    stream_ms_per_step = phases.get("probe_count_stream", (0.0, 0))[0] / max(1, args.steps)
+    eval_ms_per_step = phases.get("probe_count", (0.0, 0))[0] / max(1, args.steps)
+    probe_ms_per_step = stream_ms_per_step + eval_ms_per_step
+    achieved = algo_bytes / (probe_ms_per_step * 1e-3) / 1e9 if probe_ms_per_step > 0 else 0.0
 
     cpu = None
     if not args.no_cpu:
@@ -440,15 +421,11 @@ def main():
         "phase_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in phases.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
-                     "kernel": "k_eval (probe and count of %d of the %d join groups; HIP events around each "
-                               "launch on the context stream)" % (eval_groups, num_groups),
-                     "algorithmic_bytes_per_step": eval_bytes},
-        "roofline_single_key": {"bound": "hbm", "achieved": sk_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": sk_achieved / HBM_PEAK_GBS, "traffic": None,
-                                "kernel": "k_scalar_units (the %d join groups of units keyed on one column; HIP "
-                                          "events around each launch)" % sk_groups,
-                                "algorithmic_bytes_per_step": sk_bytes, "ms_per_step": sk_ms_per_step},
-        "probe_phase_ms_per_step": probe_ms_per_step + sk_ms_per_step,
+                     "kernel": "k_stream + k_eval (probe and count of all %d join groups; HIP events around each "
+                               "launch on the context stream)" % num_groups,
+                     "algorithmic_bytes_per_step": algo_bytes, "k_stream_ms_per_step": stream_ms_per_step,
+                     "k_eval_ms_per_step": eval_ms_per_step},
+        "probe_phase_ms_per_step": probe_ms_per_step,
         "roofline_partition": {"bound": "hbm", "achieved": part_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": part_achieved / HBM_PEAK_GBS, "traffic": None,
                                "kernel": "binding radix partition per join variable (histogram, scan, scatter, "

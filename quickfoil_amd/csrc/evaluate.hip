@@ -223,6 +223,13 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v) {
 __device__ inline uint64_t wave_sum(uint64_t v) { return wave_sum_u64(v); }
 __device__ inline uint64_t wave_sum(uint32_t v) { return wave_sum_u32(v); }
 
+// Lane i takes lane i - kShr of its row of 16 lanes (lanes shifted in from outside the row read
+// 0).
+template <int kShr>
+__device__ inline int32_t row_shr(int32_t x) {
+  return __builtin_amdgcn_update_dpp(0, x, 0x110 + kShr, 0xf, 0xf, false);  // lanes shifted in read 0
+}
+
 // ------------------------------------------------------------------------------------------
 // HBM tables for heavy partitions, per (relation, join column, subset).
 
@@ -293,6 +300,7 @@ struct DevBatch {
   int shift;  // binding digits are `shift` bits finer
   int ngroups;
   int nunits;
+  int heavy_only;  // streamed units: only the cells the streamed form leaves (light[q] == 0)
   const int32_t* tcols[EV_MAXG][EV_MAXTC + 1];  // binding table partitioned on the group's variable; [ntc] = row id
   const int64_t* toff[EV_MAXG];
   int unit[EV_MAXBU];
@@ -315,12 +323,16 @@ __device__ inline int find_unit(const int64_t* unit_cells, int num_units, int64_
 }
 
 __global__ void k_count_items(const DevBatch* batches, const DevUnit* units, const int64_t* unit_cells,
-                              int num_units, int tile, int64_t* counts) {
+                              int num_units, int tile, const uint8_t* light, int64_t* counts) {
   const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= unit_cells[num_units]) return;
   const int b = find_unit(unit_cells, num_units, cell);
   const DevBatch& B = batches[b];
   const int64_t q = cell - unit_cells[b];
+  if (B.heavy_only && light[q]) {  // the streamed form counts this cell (its batches have shift 0)
+    counts[cell] = 0;
+    return;
+  }
   bool facts = false;
   for (int k = 0; k < B.nunits; ++k) {
     const DevUnit& U = units[B.unit[k]];
@@ -333,227 +345,496 @@ __global__ void k_count_items(const DevBatch* batches, const DevUnit* units, con
 }
 
 // ------------------------------------------------------------------------------------------
-// Single-key units: a (relation, join column) all of whose literals key on the join column alone
-// (in the step, a relation's last column).  Their counts need only each binding row's number of
-// facts with its join value, cnt(r) = #{s : s[c] = r[v]}, so per value x of a cell:
-//   binding_pos += pos(x) * cnt(x),   covered_pos += pos(x) if cnt(x) > 0,
-// with pos(x) the positive binding rows with value x (likewise negatives).  The relation side is
-// reduced once to runs (value, count) per partition; a work item (cell, chunk of at most SC_ROWS
-// rows of each join variable) counts its rows' values in LDS tables, one per join variable, and
-// streams every such unit's runs of the cell through them.  A run is a distinct value, so each
-// table entry a unit's runs hit adds its rows to the covered counts once.
+// Streamed units: counting from the relation side.  Within one binding cell the binding side is
+// small (the rows of every join variable whose join value falls in the cell) and shared by every
+// unit, while a relation's facts there can be many.  A work item (cell q, chunk of binding rows,
+// part of the cell's relation entries) stages its rows once and builds two LDS tables of them:
+//  * the value table: (join value x, join variable v) -> positive and negative rows;
+//  * the pair table: one entry (row r, partner column w) per staged row of join variable v and
+//    each binding column w that some literal binds beside v, keyed (T_v(r), T_w(r)).
+// Each relation side is reduced once (cold, cached) to its distinct key tuples with their
+// multiplicities, per key subset of the unit and per cell (a subset of one column gives runs of
+// equal join values).  Every entry of the cell then probes once, whatever the clause variables:
+//  * a run (x, c) of unit u adds, for every join variable v with literals keyed on the join
+//    column alone, binding += c * rows(x, v) and covered += rows(x, v) (split by label);
+//  * a tuple t = (P_j, P_k1, ...) with count c probes (P_j, P_k1); every matching entry (r, w)
+//    with r in the rows of join variable v is a match of the literals of (u, subset, group of v)
+//    whose column k1 is bound to w and whose further columns equal row r: binding += c and
+//    covered += 1 (distinct tuples, so each (literal, row) pair matches at most one tuple).
+// Cells whose binding rows exceed TS_CMAX chunks are left to k_eval.
 
-// Run heads of a column sorted by mix32 (equal values adjacent): flags[i] = 1 at a new value.
-__global__ void k_run_heads(const int32_t* v, int64_t n, int64_t* flags) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flags[i] = (i == 0 || v[i] != v[i - 1]) ? 1 : 0;
-}
+constexpr int TS_THREADS = 512;
+constexpr int TS_WAVES = TS_THREADS / 64;
+constexpr int TS_ROWS = 1024;          // staged binding rows per work item (all join variables)
+constexpr int TS_VCAP = 2048;          // value-table slots (at most TS_ROWS entries: load <= 1/2)
+constexpr int TS_VLOG = 11;
+constexpr int TS_PCAP = 8192;          // pair-table slots
+constexpr int TS_PLOG = 13;
+constexpr int TS_PMAX = TS_PCAP / 2;   // pair entries per item (load <= 1/2)
+constexpr int TS_MAXL = 128;           // entry lists per launch
+constexpr int TS_MAXT = 256;           // (unit, join variable) counters of join-column literals per launch
+constexpr int TS_PART = 512;           // wave chunks of relation entries per work item
+constexpr int TS_CMAX = 4;             // binding chunks of a cell that the streamed form takes
+constexpr int TS_RUNCH = 16;           // runs per wave chunk: 16-lane rows, one join variable each
+constexpr int TS_TUPCH = 256;          // tuples per wave chunk (four per lane)
+constexpr int TS_STAGE = 8;            // binding words a thread stages per round (loads in flight)
+static_assert(TS_VCAP == (1 << TS_VLOG) && TS_PCAP == (1 << TS_PLOG), "table sizes are powers of two");
+static_assert(TS_ROWS <= 2047 && TS_ROWS <= 0xffff && EV_MAXTC <= 16 && EV_MAXG <= 16 && 2 * EV_MAXG < 256,
+              "slot words hold rows, columns and variables; row flags hold the variable");
+static_assert(TS_VCAP >= 2 * TS_ROWS, "value table at load <= 1/2");
 
-// Runs from the scanned heads (starts[i] = heads before row i): each run's value and first row,
-// and per partition its first run (partition p starts at row poff[p]; a run never crosses a
-// partition because equal values share their partition).  Threads cover max(n, parts + 1).
-__global__ void k_run_compact(const int32_t* v, int64_t n, const int64_t* flags, const int64_t* starts,
-                              int32_t* run_val, int32_t* run_row, const int64_t* poff, int64_t parts,
-                              int64_t* run_off) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && flags[i]) {
-    run_val[starts[i]] = v[i];
-    run_row[starts[i]] = (int32_t)i;
-  }
-  if (i <= parts) {
-    const int64_t row = poff[i];
-    run_off[i] = row < n ? starts[row] : (n > 0 ? starts[n - 1] + flags[n - 1] : 0);
-  }
-}
-
-// Run lengths from consecutive first rows; the last run ends at row n.
-__global__ void k_run_lengths(const int32_t* run_row, int64_t runs, int64_t n, int32_t* run_cnt) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < runs) run_cnt[r] = (int32_t)((r + 1 < runs ? (int64_t)run_row[r + 1] : n) - run_row[r]);
-}
-
-constexpr int SC_THREADS = 256;
-constexpr int SC_VARS = 4;     // join variables whose tables one pass holds
-constexpr int SC_ROWS = 256;   // binding rows of each join variable per work item
-constexpr int SC_CAP = 512;    // table slots per join variable (load at most 1/2)
-constexpr int SC_UNITS = 32;   // single-key units per launch (bits of a flag word)
-
-struct ScalarUnit {
-  const int32_t* run_val;  // the relation's runs on the join column, partition by partition
-  const int32_t* run_cnt;
-  const int64_t* run_off;  // per cell: first run
-  int lit[EV_MAXG][2];     // per join variable of the step: its literals [lit[v][0], lit[v][1])
+// Distinct key tuples of a (relation, join column) partition for one key subset, in cell order.
+struct TsList {
+  const int32_t* col[EV_MAXK];  // col[0]: join column value; col[1]: the pair partner; then the rest
+  const int32_t* cnt;           // facts with the tuple
+  const int64_t* off;           // per cell: first entry (cells + 1)
+  int m;                        // key columns (1: runs of the join column)
+  int unit;                     // DevUnit of the literal ranges (m >= 2)
+  int subset;                   // the unit's subset (m >= 2)
+  int kidx[EV_MAXK];            // list column i is key position kidx[i] of the subset's literals
+  int target[EV_MAXG];          // m == 1: per join variable, its counter or -1
+  int vgroup[EV_MAXG];          // m >= 2: per join variable, the unit's group or -1
 };
 
-struct ScalarArgs {
-  const ScalarUnit* units;
-  int nunits;
-  int nvars;
-  const int32_t* vals[EV_MAXG];  // join variable v's values in its partition's row order
-  const int32_t* tids[EV_MAXG];  // and the rows' ids
-  const int64_t* toff[EV_MAXG];  // cell starts of that partition
+struct TsTarget {
+  int l0, l1;  // literals [l0, l1) receive the counter
+};
+
+struct TsLaunch {
+  const TsList* lists;
+  int nlists;
+  const TsTarget* targets;
+  int ntargets;
+  int nvars;   // join variables (staged segments)
+  int ntc;     // staged binding columns
+  int tr;      // rows of each join variable per work item
+  int jt[EV_MAXG];          // staged column of each join variable
+  uint32_t wmask[EV_MAXG];  // per join variable: the staged columns paired with it
+  const int32_t* tcols[EV_MAXG][EV_MAXTC + 1];  // binding partitioned on each join variable; [ntc] = row id
+  const int64_t* toff[EV_MAXG];
   int64_t cells;
-  int num_positive;
+};
+
+__host__ __device__ inline int ts_chunk(int m) { return m == 1 ? TS_RUNCH : TS_TUPCH; }
+
+// Work items per cell: (binding chunks) x (parts of the cell's entries); light[q] = 1 when the
+// cell's binding rows fit TS_CMAX chunks (k_eval then skips it).
+__global__ void k_ts_count(const TsLaunch* L_, int64_t* counts, uint8_t* light) {
+  cptr<TsLaunch> L = as_const(L_);
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L->cells) return;
+  int64_t chunks = 0;
+  for (int v = 0; v < L->nvars; ++v) {
+    const int64_t rows = L->toff[v][q + 1] - L->toff[v][q];
+    chunks = max(chunks, (rows + L->tr - 1) / L->tr);
+  }
+  const bool lt = chunks <= TS_CMAX;
+  if (light) light[q] = lt ? 1 : 0;
+  int64_t wc = 0;
+  if (lt && chunks > 0) {
+    cptr<TsList> lists = as_const(L->lists);
+    for (int l = 0; l < L->nlists; ++l) {
+      const int64_t n = lists[l].off[q + 1] - lists[l].off[q];
+      const int csz = ts_chunk(lists[l].m);
+      wc += (n + csz - 1) / csz;
+    }
+  }
+  counts[q] = chunks * ((wc + TS_PART - 1) / TS_PART);
+}
+
+__device__ inline uint32_t ts_vhash(int32_t x, int v) { return ((uint32_t)x * 0x9e3779b1u + (uint32_t)v) * 0x85ebca6bu; }
+__device__ inline uint32_t ts_phash(int32_t x, int32_t y) {
+  return (((uint32_t)x * 0x9e3779b1u) ^ (uint32_t)y) * 0x85ebca6bu;
+}
+// Value slot word: fingerprint (17 bits) << 15 | join variable << 11 | (1 + representative row).
+// Pair slot word: fingerprint (13 bits) << 19 | join variable << 15 | partner column << 11 |
+// (1 + row).  0 = empty.
+constexpr uint32_t TS_VKEY_MASK = ~0x7ffu;
+constexpr uint32_t TS_PFP_MASK = ~((1u << 19) - 1u);
+
+struct TsArgs {
+  const TsLaunch* launch;
+  const DevUnit* units;
   const DevLit* lits;
   const DevItem* items;
   const int64_t* num_items;
-  unsigned long long* counts_out;  // ncopies x num_slots x 4
+  int64_t max_items;  // the item list's capacity
+  int num_positive;
+  unsigned long long* counts_out;  // ncopies x num_slots x 4: block b adds into copy b % ncopies
   int64_t copy_words;
   int ncopies;
 };
 
-// Items per cell: the chunks of its longest join variable's rows, none when no unit has a run
-// there.
-__global__ void k_scalar_count(ScalarArgs a, int64_t* counts) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= a.cells) return;
-  bool runs = false;
-  for (int u = 0; u < a.nunits; ++u) runs = runs || a.units[u].run_off[q + 1] > a.units[u].run_off[q];
-  int64_t rows = 0;
-  if (runs)
-    for (int v = 0; v < a.nvars; ++v) rows = max(rows, a.toff[v][q + 1] - a.toff[v][q]);
-  counts[q] = (rows + SC_ROWS - 1) / SC_ROWS;
+// A tuple matched staged row r of join variable vj through partner column w: the literals of
+// the list's subset in the unit's group of vj whose column k1 is bound to w and whose further
+// key columns equal the row's.
+__device__ __attribute__((noinline)) void ts_tuple_match(const TsArgs& a, cptr<TsList> D, int64_t pos, int r, int vj,
+                                                         int w, const int32_t* tstage, const uint8_t* rflag,
+                                                         unsigned long long* copy) {
+  const int g = D->vgroup[vj];
+  if (g < 0) return;
+  cptr<DevUnit> U = as_const(a.units) + D->unit;
+  const int l0 = U->g[g].sbeg[D->subset], l1 = U->g[g].sbeg[D->subset + 1];
+  cptr<DevLit> lits = as_const(a.lits);
+  const int m = D->m;
+  for (int li = l0; li < l1; ++li) {
+    if (lits[li].tcol[D->kidx[1]] != w) continue;
+    bool ok = true;
+    for (int i = 2; i < m && ok; ++i) ok = D->col[i][pos] == tstage[lits[li].tcol[D->kidx[i]] * TS_ROWS + r];
+    if (!ok) continue;
+    const bool p = (rflag[r] & 1) != 0;
+    unsigned long long* c = copy + (int64_t)lits[li].slot * 4;
+    atomicAdd(c + (p ? 0 : 1), 1ull);
+    atomicAdd(c + (p ? 2 : 3), (unsigned long long)(uint32_t)D->cnt[pos]);
+  }
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scalar_units(ScalarArgs a) {
-  __shared__ uint32_t word[SC_VARS][SC_CAP];  // 0 empty, else fingerprint | (1 + staged row)
-  __shared__ uint32_t pn[SC_VARS][SC_CAP];    // the value's positive rows << 16 | negative rows
-  __shared__ uint32_t hit[SC_VARS][SC_CAP];   // bit u: a run of unit u has the value
-  __shared__ int32_t sval[SC_VARS][SC_ROWS];
-  __shared__ uint8_t spos[SC_VARS][SC_ROWS];
-  __shared__ unsigned long long acc[SC_UNITS][SC_VARS][4];
-  __shared__ int64_t ubeg[SC_UNITS];      // per unit: its first run in the item's cell
-  __shared__ int ustart[SC_UNITS + 1];    // per unit: its first position among the cell's runs
-  __shared__ uint32_t uvars[SC_UNITS];    // per unit: bit j set when it has literals on variable j
-  static_assert(SC_ROWS < (1 << 16) && SC_ROWS <= (int)EV_FIELD_MASK, "row counts and rows fit their fields");
-  static_assert(SC_UNITS <= 64, "one lane per unit");
+// Inclusive sums within each row of 16 lanes (DPP row shifts); lane 15 of a row ends with the
+// row's total.  Every lane must be active.
+__device__ inline uint32_t row_sum16(uint32_t v) {
+  v += (uint32_t)row_shr<1>((int32_t)v);
+  v += (uint32_t)row_shr<2>((int32_t)v);
+  v += (uint32_t)row_shr<4>((int32_t)v);
+  v += (uint32_t)row_shr<8>((int32_t)v);
+  return v;
+}
+
+// A tuple whose first two columns are equal (x == y) matches rows of join variable vj whose
+// literal binds column k1 to vj itself: the pair table leaves those pairs out (a cell holds few
+// distinct join values, so they would pile up on few slots), and the rows of vj with value x are
+// found by scanning vj's staged rows.
+__device__ __attribute__((noinline)) void ts_equal_pair(const TsArgs& a, cptr<TsList> D, int64_t pos, int32_t x,
+                                                        int nvars, const int* seg_off, const int32_t* tstage,
+                                                        const uint8_t* rflag, const int* jt, unsigned long long* copy) {
+  for (int vj = 0; vj < nvars; ++vj) {
+    if (D->vgroup[vj] < 0) continue;
+    const int32_t* col = tstage + jt[vj] * TS_ROWS;
+    for (int r = seg_off[vj]; r < seg_off[vj + 1]; ++r)
+      if (col[r] == x) ts_tuple_match(a, D, pos, r, vj, jt[vj], tstage, rflag, copy);
+  }
+}
+
+// Probe of the pair table for tuple (x, y) at list position pos; every matching entry is
+// handed to ts_tuple_match.
+__device__ inline void ts_pair_probe(const TsArgs& a, cptr<TsList> D, int64_t pos, int32_t x, int32_t y,
+                                     const uint32_t* pword, const int32_t* tstage, const uint8_t* rflag, const int* jt,
+                                     unsigned long long* copy) {
+  const uint32_t h = ts_phash(x, y);
+  const uint32_t fp = ((h >> 4) & 0x1fffu) << 19;
+  uint32_t slot = h >> (32 - TS_PLOG);
+  while (true) {
+    const uint32_t wd = pword[slot];
+    if (wd == 0) break;
+    if ((wd & TS_PFP_MASK) == fp) {
+      const int r = (int)(wd & 0x7ffu) - 1, w = (int)((wd >> 11) & 15u), vj = (int)((wd >> 15) & 15u);
+      if (tstage[jt[vj] * TS_ROWS + r] == x && tstage[w * TS_ROWS + r] == y)
+        ts_tuple_match(a, D, pos, r, vj, w, tstage, rflag, copy);
+    }
+    slot = (slot + 1) & (TS_PCAP - 1);
+  }
+}
+
+__global__ __launch_bounds__(TS_THREADS) void k_stream(TsArgs a) {
+  extern __shared__ uint32_t lds[];
+  cptr<TsLaunch> L = as_const(a.launch);
+  const int ntc = L->ntc;
+  int32_t* tstage = reinterpret_cast<int32_t*>(lds);   // ntc x TS_ROWS
+  uint32_t* vword = lds + ntc * TS_ROWS;               // TS_VCAP
+  uint32_t* vpn = vword + TS_VCAP;                     // TS_VCAP: positive rows << 16 | negative rows
+  uint32_t* pword = vpn + TS_VCAP;                     // TS_PCAP
+  uint8_t* rflag = reinterpret_cast<uint8_t*>(pword + TS_PCAP);  // TS_ROWS: join variable << 1 | positive
+  __shared__ unsigned long long acc[TS_MAXT][4];
+  __shared__ const int64_t* loff[TS_MAXL];  // per list: its cell offsets
+  __shared__ int lm[TS_MAXL];
+  __shared__ int64_t lbase[TS_MAXL];        // per list: the item's cell: first entry, entries, first wave chunk
+  __shared__ int lcnt[TS_MAXL];
+  __shared__ int lch[TS_MAXL + 1];
+  __shared__ const int32_t* tcol[EV_MAXG][EV_MAXTC + 1];
+  __shared__ const int64_t* toff[EV_MAXG];
+  __shared__ int seg_off[EV_MAXG + 1];
+  __shared__ int64_t seg_src[EV_MAXG];
+  __shared__ int jt[EV_MAXG];
+  __shared__ int item_part;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int64_t total = uni64(*a.num_items);
-  cptr<ScalarUnit> units = as_const(a.units);
+  const int wave = uni(threadIdx.x >> 6);
+  const int nvars = L->nvars, nlists = L->nlists;
+  const int64_t total = min(uni64(*a.num_items), a.max_items);
+  cptr<DevItem> items = as_const(a.items);
+  cptr<TsList> lists = as_const(L->lists);
   unsigned long long* copy = a.counts_out + (int64_t)(blockIdx.x % a.ncopies) * a.copy_words;
-  // The tables start empty; the pass over them that ends every item empties them again.
-  for (int i = threadIdx.x; i < SC_VARS * SC_CAP; i += SC_THREADS) {
-    (&word[0][0])[i] = 0;
-    (&pn[0][0])[i] = 0;
-    (&hit[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < TS_MAXT * 4; i += TS_THREADS) (&acc[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < nlists; i += TS_THREADS) {
+    loff[i] = lists[i].off;
+    lm[i] = lists[i].m;
   }
-  for (int vg = 0; vg < a.nvars; vg += SC_VARS) {
-    const int nv = min(SC_VARS, a.nvars - vg);
-    for (int i = threadIdx.x; i < SC_UNITS * SC_VARS * 4; i += SC_THREADS) (&acc[0][0][0])[i] = 0;
-    for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
-      const int64_t q = a.items[it].q;
-      const int64_t r0 = (int64_t)a.items[it].chunk * SC_ROWS;
-      __syncthreads();  // the previous item is done with the staged rows and the unit map
-      int n[SC_VARS];
+  for (int i = threadIdx.x; i < nvars * (ntc + 1); i += TS_THREADS) tcol[i / (ntc + 1)][i % (ntc + 1)] = L->tcols[i / (ntc + 1)][i % (ntc + 1)];
+  if ((int)threadIdx.x < nvars) {
+    jt[threadIdx.x] = L->jt[threadIdx.x];
+    toff[threadIdx.x] = L->toff[threadIdx.x];
+  }
+
+  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const int64_t q = items[it].q;
+    const int idx = items[it].chunk;  // the item's index within its cell
+    __syncthreads();  // the previous item is done with the tables, rows and lists
+    if (wave == 0) {
+      // Entry lists of the cell (two per lane, loads in flight together with the join
+      // variables' cell bounds): first entry, entries, first wave chunk.
+      int64_t b0[2] = {0, 0}, b1[2] = {0, 0};
 #pragma unroll
-      for (int j = 0; j < SC_VARS; ++j) {
-        n[j] = 0;
-        if (j < nv) {
-          const int64_t t0 = a.toff[vg + j][q], t1 = a.toff[vg + j][q + 1];
-          n[j] = (int)max<int64_t>(0, min<int64_t>(SC_ROWS, t1 - t0 - r0));
-          for (int r = threadIdx.x; r < n[j]; r += SC_THREADS) {
-            sval[j][r] = a.vals[vg + j][t0 + r0 + r];
-            spos[j][r] = a.tids[vg + j][t0 + r0 + r] < a.num_positive ? 1 : 0;
-          }
+      for (int j = 0; j < 2; ++j) {
+        const int l = lane + 64 * j;
+        if (l < nlists) {
+          b0[j] = loff[l][q];
+          b1[j] = loff[l][q + 1];
         }
       }
-      if (wave == 0) {  // the cell's runs, unit by unit (one lane per unit)
-        int cnt = 0;
-        uint32_t vars = 0;
-        if (lane < a.nunits) {
-          cptr<ScalarUnit> U = units + lane;
-          const int64_t b0 = U->run_off[q];
-          cnt = (int)(U->run_off[q + 1] - b0);
-          ubeg[lane] = b0;
-          for (int j = 0; j < nv; ++j)
-            if (n[j] > 0 && U->lit[vg + j][1] > U->lit[vg + j][0]) vars |= 1u << j;
-          uvars[lane] = vars;
-          if (vars == 0) cnt = 0;
+      int64_t t0 = 0, t1 = 0;
+      if (lane < nvars) {
+        t0 = toff[lane][q];
+        t1 = toff[lane][q + 1];
+      }
+      int carry = 0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int l = lane + 64 * j;
+        int ch = 0;
+        if (l < nlists) {
+          const int n = (int)(b1[j] - b0[j]);
+          const int csz = ts_chunk(lm[l]);
+          ch = (n + csz - 1) / csz;
+          lbase[l] = b0[j];
+          lcnt[l] = n;
         }
-        int x = cnt;  // inclusive prefix over the wave
+        int x = ch;
         for (int off = 1; off < 64; off <<= 1) {
           const int y = __shfl_up(x, off, 64);
           if (lane >= off) x += y;
         }
-        if (lane < a.nunits) ustart[lane] = x - cnt;
-        if (lane == a.nunits - 1) ustart[a.nunits] = x;
+        if (l < nlists) lch[l] = carry + x - ch;
+        carry += __shfl(x, 63, 64);
       }
-      __syncthreads();  // rows staged, unit map ready (tables empty)
-      for (int i = threadIdx.x; i < SC_VARS * SC_ROWS; i += SC_THREADS) {
-        const int j = i / SC_ROWS, r = i - j * SC_ROWS;
-        if (j >= nv || r >= n[j]) continue;
-        const int32_t x = sval[j][r];
-        const uint32_t h = lds_hash(&x, 1), fp = lds_fp(h);
-        uint32_t slot = lds_slot(h, SC_CAP);
-        while (true) {
-          const uint32_t prev = atomicCAS(&word[j][slot], 0u, fp | ((uint32_t)(r + 1) << EV_REP_SHIFT));
-          if (prev == 0) break;
-          if ((prev & EV_FP_MASK) == fp && sval[j][((prev >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1] == x) break;
-          slot = (slot + 1) & (SC_CAP - 1);
-        }
-        atomicAdd(&pn[j][slot], spos[j][r] ? (1u << 16) : 1u);
+      const int parts = (carry + TS_PART - 1) / TS_PART;  // >= 1: the cell has items
+      const int chunk = idx / parts;
+      // Binding rows of the chunk, one lane per join variable.
+      int n = 0;
+      int64_t s0 = 0;
+      if (lane < nvars) {
+        s0 = t0 + (int64_t)chunk * L->tr;
+        n = (int)max<int64_t>(0, min<int64_t>(L->tr, t1 - s0));
       }
-      __syncthreads();  // tables ready
-      // Every (unit, run) of the cell, one per thread at a time.
-      const int nruns = ustart[a.nunits];
-      for (int i = threadIdx.x; i < nruns; i += SC_THREADS) {
-        int u = 0;
-        for (int step = SC_UNITS / 2; step > 0; step >>= 1)
-          if (u + step < a.nunits && ustart[u + step] <= i) u += step;
-        const int64_t run = ubeg[u] + (i - ustart[u]);
-        const uint32_t vars = uvars[u];
-        const int32_t x = units[u].run_val[run];
-        const uint64_t c = (uint32_t)units[u].run_cnt[run];
-        const uint32_t h = lds_hash(&x, 1), fp = lds_fp(h);
+      int x = n;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+      }
+      if (lane < nvars) {
+        seg_off[lane] = x - n;
+        seg_src[lane] = s0;
+      }
+      if (lane == nvars - 1) seg_off[nvars] = x;
+      if (lane == 0) {
+        lch[nlists] = carry;
+        item_part = idx - chunk * parts;
+      }
+    } else {
+      // The tables start empty.
+      const int n16 = (2 * TS_VCAP + TS_PCAP) / 4;
+      uint4* t = reinterpret_cast<uint4*>(vword);
+      for (int i = threadIdx.x - 64; i < n16; i += TS_THREADS - 64) t[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    // Stage the rows: the ntc binding columns and a flag (join variable << 1 | positive, from
+    // the row id), segment after segment, column-major within a segment; each thread keeps
+    // TS_STAGE loads in flight.
+    const int rows_all = seg_off[nvars];
+    const int words_all = (ntc + 1) * rows_all;
+    for (int w0 = 0; w0 < words_all; w0 += TS_STAGE * TS_THREADS) {
+      int32_t val[TS_STAGE];
+      int dst[TS_STAGE];
 #pragma unroll
-        for (int j = 0; j < SC_VARS; ++j) {
-          if (!((vars >> j) & 1u)) continue;
-          uint32_t slot = lds_slot(h, SC_CAP);
-          while (true) {
-            const uint32_t w = word[j][slot];
-            if (w == 0) break;
-            if ((w & EV_FP_MASK) == fp && sval[j][((w >> EV_REP_SHIFT) & EV_FIELD_MASK) - 1] == x) {
-              const uint32_t rows = pn[j][slot];
-              if (rows >> 16) atomicAdd(&acc[u][j][2], (unsigned long long)(c * (rows >> 16)));
-              if (rows & 0xffffu) atomicAdd(&acc[u][j][3], (unsigned long long)(c * (rows & 0xffffu)));
-              atomicOr(&hit[j][slot], 1u << u);
-              break;
+      for (int j = 0; j < TS_STAGE; ++j) {
+        const int w = w0 + (int)threadIdx.x + j * TS_THREADS;
+        dst[j] = -1;
+        val[j] = 0;
+        if (w < words_all) {
+          int v = 0;
+          while (w >= (ntc + 1) * seg_off[v + 1]) ++v;
+          const int r0 = seg_off[v], n = seg_off[v + 1] - r0;
+          const int wi = w - (ntc + 1) * r0;
+          const int c = wi / n, i = wi - c * n;
+          val[j] = tcol[v][c][seg_src[v] + i];
+          dst[j] = c < ntc ? c * TS_ROWS + r0 + i : -2 - ((r0 + i) | (v << 16));
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TS_STAGE; ++j) {
+        if (dst[j] >= 0) {
+          tstage[dst[j]] = val[j];
+        } else if (dst[j] <= -2) {
+          const int e = -2 - dst[j];
+          rflag[e & 0xffff] = (uint8_t)(((e >> 16) << 1) | (val[j] < a.num_positive ? 1 : 0));
+        }
+      }
+    }
+    __syncthreads();
+    // Build both tables.
+    for (int r = threadIdx.x; r < rows_all; r += TS_THREADS) {
+      const int fl = rflag[r];
+      const int vj = fl >> 1;
+      const int jc = jt[vj];
+      const int32_t x = tstage[jc * TS_ROWS + r];
+      {
+        const uint32_t h = ts_vhash(x, vj);
+        const uint32_t key = (((h >> 4) & 0x1ffffu) << 15) | ((uint32_t)vj << 11);
+        uint32_t slot = h >> (32 - TS_VLOG);
+        while (true) {
+          const uint32_t prev = atomicCAS(&vword[slot], 0u, key | (uint32_t)(r + 1));
+          if (prev == 0) break;
+          if ((prev & TS_VKEY_MASK) == key && tstage[jc * TS_ROWS + (int)(prev & 0x7ffu) - 1] == x) break;
+          slot = (slot + 1) & (TS_VCAP - 1);
+        }
+        atomicAdd(&vpn[slot], (fl & 1) ? 0x10000u : 1u);
+      }
+      uint32_t wm = L->wmask[vj] & ~(1u << jc);  // pairs of the join variable with itself: see ts_equal_pair
+      while (wm) {
+        const int w = __ffs((int)wm) - 1;
+        wm &= wm - 1;
+        const uint32_t h = ts_phash(x, tstage[w * TS_ROWS + r]);
+        const uint32_t word = (((h >> 4) & 0x1fffu) << 19) | ((uint32_t)vj << 15) | ((uint32_t)w << 11) | (uint32_t)(r + 1);
+        uint32_t slot = h >> (32 - TS_PLOG);
+        while (atomicCAS(&pword[slot], 0u, word) != 0) slot = (slot + 1) & (TS_PCAP - 1);
+      }
+    }
+    __syncthreads();
+    // Stream the item's wave chunks of the cell's entries.
+    const int c_end = lch[nlists];
+    const int c0 = item_part * TS_PART, c1 = min(c0 + TS_PART, c_end);
+    int l = 0;
+    for (int k = c0 + wave; k < c1; k += TS_WAVES) {
+      while (lch[l + 1] <= k) ++l;  // lists with no chunk in this cell are passed over
+      l = uni(l);
+      cptr<TsList> D = lists + l;
+      const int n = lcnt[l];
+      const int64_t base = lbase[l];
+      if (lm[l] == 1) {
+        // Runs: lanes (join variable vg + lane / 16, run e + lane % 16).
+        const int e = (k - lch[l]) * TS_RUNCH + (lane & 15);
+        const bool in = e < n;
+        int32_t x = 0;
+        uint32_t c = 0;
+        if (in) {
+          x = D->col[0][base + e];
+          c = (uint32_t)D->cnt[base + e];
+        }
+        for (int vg = 0; vg < nvars; vg += 4) {
+          const int vj = vg + (lane >> 4);
+          const int t = vj < nvars ? D->target[vj] : -1;
+          if (!__ballot(t >= 0)) continue;
+          uint32_t pn = 0;
+          if (in && t >= 0) {
+            const uint32_t h = ts_vhash(x, vj);
+            const uint32_t key = (((h >> 4) & 0x1ffffu) << 15) | ((uint32_t)vj << 11);
+            const int jc = jt[vj];
+            uint32_t slot = h >> (32 - TS_VLOG);
+            while (true) {
+              const uint32_t wd = vword[slot];
+              if (wd == 0) break;
+              if ((wd & TS_VKEY_MASK) == key && tstage[jc * TS_ROWS + (int)(wd & 0x7ffu) - 1] == x) {
+                pn = vpn[slot];
+                break;
+              }
+              slot = (slot + 1) & (TS_VCAP - 1);
             }
-            slot = (slot + 1) & (SC_CAP - 1);
+          }
+          // Row sums: covered (packed; a row's runs are distinct values, so its positive and
+          // negative rows stay below the chunk's rows) and binding in 21-bit pieces (a run count
+          // is below 2^31 and rows below 2^11, so each piece's sum over 16 lanes fits 32 bits).
+          const uint64_t bp = (uint64_t)c * (pn >> 16), bn = (uint64_t)c * (pn & 0xffffu);
+          const uint32_t cov = row_sum16(pn);
+          const uint32_t bp_lo = row_sum16((uint32_t)(bp & 0x1fffffu)), bp_hi = row_sum16((uint32_t)(bp >> 21));
+          const uint32_t bn_lo = row_sum16((uint32_t)(bn & 0x1fffffu)), bn_hi = row_sum16((uint32_t)(bn >> 21));
+          if ((lane & 15) == 15 && t >= 0 && cov != 0) {
+            atomicAdd(&acc[t][0], (unsigned long long)(cov >> 16));
+            atomicAdd(&acc[t][1], (unsigned long long)(cov & 0xffffu));
+            atomicAdd(&acc[t][2], ((unsigned long long)bp_hi << 21) + bp_lo);
+            atomicAdd(&acc[t][3], ((unsigned long long)bn_hi << 21) + bn_lo);
           }
         }
-      }
-      __syncthreads();  // every unit's hits recorded
-      // Covered rows: each value a unit's runs hit, once; the slots are emptied for the next item.
-      for (int i = threadIdx.x; i < SC_VARS * SC_CAP; i += SC_THREADS) {
-        if ((&word[0][0])[i] == 0) continue;
-        uint32_t f = (&hit[0][0])[i];
-        const uint32_t rows = (&pn[0][0])[i];
-        (&word[0][0])[i] = 0;
-        (&pn[0][0])[i] = 0;
-        (&hit[0][0])[i] = 0;
-        const int j = i / SC_CAP;
-        while (f) {
-          const int u = __ffs((int)f) - 1;
-          f &= f - 1;
-          if (rows >> 16) atomicAdd(&acc[u][j][0], (unsigned long long)(rows >> 16));
-          if (rows & 0xffffu) atomicAdd(&acc[u][j][1], (unsigned long long)(rows & 0xffffu));
+      } else {
+        // Tuples: TS_TUPCH / 64 per lane, their loads in flight together.
+        constexpr int J = TS_TUPCH / 64;
+        const int e0 = (k - lch[l]) * TS_TUPCH + lane;
+        const int32_t* c0p = D->col[0];
+        const int32_t* c1p = D->col[1];
+        int32_t x[J], y[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int e = e0 + 64 * j;
+          x[j] = y[j] = 0;
+          if (e < n) {
+            x[j] = c0p[base + e];
+            y[j] = c1p[base + e];
+          }
         }
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+          if (e0 + 64 * j < n) {
+            ts_pair_probe(a, D, base + e0 + 64 * j, x[j], y[j], pword, tstage, rflag, jt, copy);
+            if (x[j] == y[j]) ts_equal_pair(a, D, base + e0 + 64 * j, x[j], nvars, seg_off, tstage, rflag, jt, copy);
+          }
       }
     }
-    __syncthreads();  // this pass's counts complete
-    cptr<DevLit> lits = as_const(a.lits);
-    for (int i = threadIdx.x; i < a.nunits * nv * 4; i += SC_THREADS) {
-      const int u = i / (nv * 4), j = (i / 4) % nv, c = i % 4;
-      const unsigned long long v = acc[u][j][c];
-      if (v == 0) continue;
-      for (int li = units[u].lit[vg + j][0]; li < units[u].lit[vg + j][1]; ++li)
-        atomicAdd(copy + (int64_t)lits[li].slot * 4 + c, v);
-    }
-    __syncthreads();  // before the next pass clears acc
   }
+  __syncthreads();  // every item's counters are in acc
+  cptr<TsTarget> targets = as_const(L->targets);
+  cptr<DevLit> lits = as_const(a.lits);
+  for (int i = threadIdx.x; i < L->ntargets * 4; i += TS_THREADS) {
+    const unsigned long long v = acc[i >> 2][i & 3];
+    if (v == 0) continue;
+    for (int li = targets[i >> 2].l0; li < targets[i >> 2].l1; ++li)
+      atomicAdd(copy + (int64_t)lits[li].slot * 4 + (i & 3), v);
+  }
+}
+
+// Distinct tuples of sorted rows: heads[i] = 1 where a new tuple starts.
+struct TsCols {
+  const int32_t* c[EV_MAXK];
+  int m;
+};
+__global__ void k_tuple_heads(TsCols s, int64_t n, int64_t* flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool head = i == 0;
+  for (int k = 0; k < s.m && !head; ++k) head = s.c[k][i] != s.c[k][i - 1];
+  flags[i] = head ? 1 : 0;
+}
+
+// Tuples from the scanned heads (starts[i] = heads before row i): each tuple's columns and first
+// row, and per cell its first tuple (cell p starts at row poff[p]; equal tuples share their
+// cell).  Threads cover max(n, cells + 1).
+struct TsOut {
+  int32_t* c[EV_MAXK];
+};
+__global__ void k_tuple_compact(TsCols s, int64_t n, const int64_t* flags, const int64_t* starts, TsOut out,
+                                int32_t* first_row, const int64_t* poff, int64_t cells, int64_t* tuple_off) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flags[i]) {
+    const int64_t t = starts[i];
+    for (int k = 0; k < s.m; ++k) out.c[k][t] = s.c[k][i];
+    first_row[t] = (int32_t)i;
+  }
+  if (i <= cells) {
+    const int64_t row = poff[i];
+    tuple_off[i] = row < n ? starts[row] : (n > 0 ? starts[n - 1] + flags[n - 1] : 0);
+  }
+}
+
+// Tuple multiplicities from consecutive first rows; the last tuple ends at row n.
+__global__ void k_tuple_counts(const int32_t* first_row, int64_t tuples, int64_t n, int32_t* cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < tuples) cnt[t] = (int32_t)((t + 1 < tuples ? (int64_t)first_row[t + 1] : n) - first_row[t]);
 }
 
 // One thread per work item (the grid covers an upper bound of the item count): item i belongs to
@@ -960,10 +1241,6 @@ __device__ inline void eval_subset(const EvalArgs& a, cptr<DevUnit> U, ItemShare
 // lane 15 ends with its exclusive sum, so lane 15's value plus its own is the total.  Lanes 16-63
 // hold sums over their own rows, which callers ignore.  Every lane of the wave must be active.
 static_assert(EV_MAXG <= 16, "segments fit one DPP row");
-template <int kShr>
-__device__ inline int32_t row_shr(int32_t x) {
-  return __builtin_amdgcn_update_dpp(0, x, 0x110 + kShr, 0xf, 0xf, false);  // lanes shifted in read 0
-}
 __device__ inline int row_excl_prefix(int v) {
   int x = v;
   x += row_shr<1>(x);
@@ -1040,7 +1317,6 @@ __global__ __launch_bounds__(EV_THREADS) __attribute__((amdgpu_waves_per_eu(QF_E
   const int64_t total = uni64(*a.num_items);
   const int ntc = a.ntc;
   cptr<DevItem> items = as_const(a.items);
-  cptr<DevLit> lits = as_const(a.lits);
   FactPrefetch pf;
   // This block's copy of the counters: blocks spread their atomics over separate copies, so that
   // few of them queue on one address.
@@ -1325,10 +1601,13 @@ struct HeavyTables {
                 // table, so that nothing waits for the build kernels)
 };
 
-// A relation's join column as runs of equal values, partition by partition (single-key units).
-struct Runs {
-  DevBuf val, cnt;  // int32 per run
-  DevBuf off;       // int64 per partition + 1: first run
+// A relation partition's distinct key tuples for one key subset (join column first), cell by
+// cell, with their multiplicities (streamed units).
+struct TupleList {
+  std::vector<DevBuf> cols;  // int32 per tuple and key column
+  DevBuf cnt;                // int32 per tuple: facts with the tuple
+  DevBuf off;                // int64 per cell + 1: first tuple
+  int64_t tuples = 0;
 };
 
 // A relation partitioned on one join column, with its heavy partitions and their tables.
@@ -1338,7 +1617,7 @@ struct RelPart {
   DevBuf hq;                     // int32 per partition
   std::vector<int64_t> heavy_q;  // heavy partitions
   std::map<std::vector<int>, std::shared_ptr<HeavyTables>> tables;
-  std::shared_ptr<Runs> runs;
+  std::map<std::vector<int>, std::shared_ptr<TupleList>> tuples;
 };
 
 struct RelCache {
@@ -1392,46 +1671,68 @@ static std::shared_ptr<RelPart> partition_relation(Context* ctx, const qf_table&
   return rp;
 }
 
-// The partitioned join column sorted by all 32 bits of mix32 (its top radix bits are the
-// partition, so every partition keeps its rows) and cut into runs of equal values; made once per
-// relation partition and cached with it.
-static std::shared_ptr<Runs> relation_runs(Context* ctx, RelPart& rp) {
-  if (rp.runs) return rp.runs;
-  PhaseScope ps(ctx, "heavy_tables");
+// The partition's rows sorted by the key columns (each by all 32 bits of its mix32; the join
+// column, key[0], last, so that its top radix bits keep every cell's rows in place) and cut into
+// distinct tuples; made once per (relation partition, key) and cached with the partition.
+static std::shared_ptr<TupleList> relation_tuples(Context* ctx, RelPart& rp, const std::vector<int>& key) {
+  auto it = rp.tuples.find(key);
+  if (it != rp.tuples.end()) return it->second;
+  PhaseScope ps(ctx, "relation_tuples");
+  const int m = (int)key.size();
+  QF_REQUIRE(m >= 1 && m <= EV_MAXK && key[0] == rp.join_col, "bad tuple key");
   const int64_t n = rp.pt->num_rows;
   const int64_t parts = (int64_t)1 << rp.pt->radix_bits;
-  auto ru = std::make_shared<Runs>();
-  DevBuf sorted(ctx, std::max<int64_t>(1, n) * 4), flags(ctx, std::max<int64_t>(1, n) * 8),
-      starts(ctx, std::max<int64_t>(1, n) * 8);
-  const int32_t* in = rp.pt->columns[rp.join_col].as<int32_t>();
-  int32_t* out = sorted.as<int32_t>();
-  stable_partition(ctx, &in, 1, 0, false, n, 1, 32, &out, 1, nullptr);
-  int64_t runs = 0;
+  auto tl = std::make_shared<TupleList>();
+  std::vector<DevBuf> buf[2];
+  for (int s = 0; s < 2; ++s)
+    for (int k = 0; k < m; ++k) buf[s].emplace_back(ctx, std::max<int64_t>(1, n) * 4);
+  std::vector<const int32_t*> cur(m);
+  for (int k = 0; k < m; ++k) cur[k] = rp.pt->columns[key[k]].as<int32_t>();
   if (n > 0) {
-    hipLaunchKernelGGL(k_run_heads, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, out, n,
+    int side = 0;
+    for (int k = m - 1; k >= 0; --k) {  // stable LSD passes, the join column last
+      std::vector<int32_t*> out(m);
+      for (int c = 0; c < m; ++c) out[c] = buf[side][c].as<int32_t>();
+      stable_partition(ctx, cur.data(), m, k, false, n, 1, 32, out.data(), 1, nullptr);
+      for (int c = 0; c < m; ++c) cur[c] = out[c];
+      side ^= 1;
+    }
+  }
+  TsCols sc{};
+  sc.m = m;
+  for (int k = 0; k < m; ++k) sc.c[k] = cur[k];
+  DevBuf flags(ctx, std::max<int64_t>(1, n) * 8), starts(ctx, std::max<int64_t>(1, n) * 8);
+  int64_t tuples = 0;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_tuple_heads, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, sc, n,
                        flags.as<int64_t>());
     QF_CHECK_HIP(hipGetLastError());
     scan_exclusive_i64(ctx, flags.as<int64_t>(), starts.as<int64_t>(), n);
     int64_t last[2];
     copy_d2h(ctx, &last[0], starts.as<int64_t>() + n - 1, 8);
     copy_d2h(ctx, &last[1], flags.as<int64_t>() + n - 1, 8);
-    runs = last[0] + last[1];
+    tuples = last[0] + last[1];
   }
-  ru->val = DevBuf(ctx, std::max<int64_t>(1, runs) * 4);
-  ru->cnt = DevBuf(ctx, std::max<int64_t>(1, runs) * 4);
-  ru->off = DevBuf(ctx, (parts + 1) * 8);
-  DevBuf row(ctx, std::max<int64_t>(1, runs) * 4);
-  hipLaunchKernelGGL(k_run_compact, dim3((unsigned)ceil_div(std::max(n, parts + 1), 256)), dim3(256), 0, ctx->stream,
-                     out, n, flags.as<int64_t>(), starts.as<int64_t>(), ru->val.as<int32_t>(), row.as<int32_t>(),
-                     rp.pt->offsets.as<int64_t>(), parts, ru->off.as<int64_t>());
+  TsOut out{};
+  for (int k = 0; k < m; ++k) {
+    tl->cols.emplace_back(ctx, std::max<int64_t>(1, tuples) * 4);
+    out.c[k] = tl->cols.back().as<int32_t>();
+  }
+  tl->cnt = DevBuf(ctx, std::max<int64_t>(1, tuples) * 4);
+  tl->off = DevBuf(ctx, (parts + 1) * 8);
+  tl->tuples = tuples;
+  DevBuf first(ctx, std::max<int64_t>(1, tuples) * 4);
+  hipLaunchKernelGGL(k_tuple_compact, dim3((unsigned)ceil_div(std::max(n, parts + 1), 256)), dim3(256), 0,
+                     ctx->stream, sc, n, flags.as<int64_t>(), starts.as<int64_t>(), out, first.as<int32_t>(),
+                     rp.pt->offsets.as<int64_t>(), parts, tl->off.as<int64_t>());
   QF_CHECK_HIP(hipGetLastError());
-  if (runs > 0) {
-    hipLaunchKernelGGL(k_run_lengths, dim3((unsigned)ceil_div(runs, 256)), dim3(256), 0, ctx->stream,
-                       row.as<int32_t>(), runs, n, ru->cnt.as<int32_t>());
+  if (tuples > 0) {
+    hipLaunchKernelGGL(k_tuple_counts, dim3((unsigned)ceil_div(tuples, 256)), dim3(256), 0, ctx->stream,
+                       first.as<int32_t>(), tuples, n, tl->cnt.as<int32_t>());
     QF_CHECK_HIP(hipGetLastError());
   }
-  rp.runs = ru;
-  return ru;
+  rp.tuples[key] = tl;
+  return tl;
 }
 
 static std::shared_ptr<HeavyTables> heavy_tables(Context* ctx, RelPart& rp, const std::vector<int>& pcols) {
@@ -1524,11 +1825,17 @@ struct StepPlan {
   std::vector<std::shared_ptr<RelPart>> keep_rel;
   std::vector<std::shared_ptr<HeavyTables>> keep_heavy;
   std::vector<DevUnit> units;
-  std::vector<char> unit_scalar;            // per unit: counted by k_scalar_units instead of k_eval
-  std::vector<ScalarUnit> sunits;
-  std::vector<std::shared_ptr<Runs>> keep_runs;
-  DevBuf d_sunits, d_sc_cells, d_sc_counts, d_sc_starts, d_sc_items, d_sc_num;
-  int64_t sc_items = 0;                     // upper bound of the single-key work items
+  std::vector<char> unit_streamed;          // per unit: counted by k_stream in light cells, k_eval in the rest
+  std::vector<std::shared_ptr<TupleList>> keep_tuples;
+  struct StreamLaunch {
+    DevBuf d_launch, d_lists, d_targets;
+    int64_t items_bound = 0;
+  };
+  std::vector<StreamLaunch> slaunches;      // units of the streamed form, in launches of bounded lists
+  DevBuf d_ts_cells, d_ts_counts, d_ts_starts, d_ts_items, d_ts_num, d_light;
+  int64_t ts_items = 0;                     // upper bound of any streamed launch's work items
+  size_t ts_lds = 0;
+  int ts_blocks = 0;
   std::vector<DevBatch> batches;
   std::vector<DevLit> lits;
   std::vector<std::pair<int, int>> launches;  // unit ranges launched together
@@ -1555,11 +1862,11 @@ struct StepPlan {
 
 // Units: (relation, join column) with a group per join variable (batched), one group per
 // literal-group (plain group), or one literal each (plain literal).
-// scalar_units: units whose literals all key on the join column alone are counted from value
-// runs by k_scalar_units (the batched form without covered bitvectors).
+// stream: the batched form without covered bitvectors, whose units k_stream counts from their
+// relations' distinct key tuples in the cells whose binding rows fit its work items.
 static void plan_step(Context* ctx, const qf_table& binding, const qf_table* relations,
                       const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P,
-                      bool scalar_units) {
+                      bool stream) {
   const bool cache = o.reuse_partitions != 0;
   P.num_slots = num_slots;
   P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
@@ -1618,6 +1925,14 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     std::vector<std::vector<const LitSpec*>> lits;
   };
   std::vector<UnitSpec> us;
+  // Streamed units: their entry lists (targets numbered within the unit) and pair columns.
+  struct UnitStream {
+    std::vector<TsList> lists;
+    std::vector<int64_t> sizes;  // entries of each list
+    std::vector<TsTarget> targets;
+    uint32_t wmask[EV_MAXG] = {};
+  };
+  std::vector<UnitStream> ustreams;
   std::vector<std::vector<int>> unit_vars;  // per unit: the join variable of each of its groups
   std::map<std::pair<int, int>, int> uindex;
   for (const GroupSpec& g : specs) {
@@ -1740,26 +2055,66 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       else du.child_subsets[dl.pmask] |= 1u << dl.subset;
     }
     du.nsub = (int)subsets.size();
-    const bool scalar = scalar_units && du.nsub == 1;
-    P.unit_scalar.push_back(scalar ? 1 : 0);
-    if (scalar) {
-      auto ru = relation_runs(ctx, rp);
-      P.keep_runs.push_back(ru);
-      ScalarUnit su{};
-      su.run_val = ru->val.as<int32_t>();
-      su.run_cnt = ru->cnt.as<int32_t>();
-      su.run_off = ru->off.as<int64_t>();
-      for (int gi = 0; gi < du.ngroups; ++gi) {
-        const int vi = var_index(S.groups[gi]->join_var);
-        su.lit[vi][0] = du.g[gi].sbeg[0];
-        su.lit[vi][1] = du.g[gi].sbeg[1];
+    // The streamed form takes the unit when its cells are the binding's and no literal carries a
+    // row-side equality (the tables pair a join variable with one more column).
+    bool eqs = false;
+    for (const auto& gl : S.lits)
+      for (const LitSpec* l : gl) eqs = eqs || !l->eqs.empty();
+    const bool streamed = stream && !eqs && du.R == P.R_T;
+    P.unit_streamed.push_back(streamed ? 1 : 0);
+    if (streamed) {
+      UnitStream ust;
+      const int unit_index = (int)P.units.size();
+      TsList run{};
+      for (int v = 0; v < EV_MAXG; ++v) run.target[v] = run.vgroup[v] = -1;
+      for (int gi = 0; gi < du.ngroups; ++gi)
+        if (du.g[gi].sbeg[1] > du.g[gi].sbeg[0]) {
+          run.target[var_index(S.groups[gi]->join_var)] = (int)ust.targets.size();
+          ust.targets.push_back(TsTarget{du.g[gi].sbeg[0], du.g[gi].sbeg[1]});
+        }
+      if (!ust.targets.empty()) {
+        auto tl = relation_tuples(ctx, rp, std::vector<int>{jc});
+        P.keep_tuples.push_back(tl);
+        run.m = 1;
+        run.col[0] = tl->cols[0].as<int32_t>();
+        run.cnt = tl->cnt.as<int32_t>();
+        run.off = tl->off.as<int64_t>();
+        run.unit = unit_index;
+        ust.lists.push_back(run);
+        ust.sizes.push_back(tl->tuples);
       }
-      P.sunits.push_back(su);
+      for (int s = 1; s < (int)subsets.size(); ++s) {
+        const std::vector<int>& pc = subsets[s].first;  // ascending relation columns, the join column among them
+        std::vector<int> key{jc};
+        for (int c : pc)
+          if (c != jc) key.push_back(c);
+        auto tl = relation_tuples(ctx, rp, key);
+        P.keep_tuples.push_back(tl);
+        TsList L{};
+        L.m = (int)key.size();
+        for (int i = 0; i < L.m; ++i) {
+          L.col[i] = tl->cols[i].as<int32_t>();
+          L.kidx[i] = (int)(std::find(pc.begin(), pc.end(), key[i]) - pc.begin());
+        }
+        L.cnt = tl->cnt.as<int32_t>();
+        L.off = tl->off.as<int64_t>();
+        L.unit = unit_index;
+        L.subset = s;
+        for (int v = 0; v < EV_MAXG; ++v) L.target[v] = L.vgroup[v] = -1;
+        for (int gi = 0; gi < du.ngroups; ++gi) {
+          const int vi = var_index(S.groups[gi]->join_var);
+          L.vgroup[vi] = gi;
+          for (int li = du.g[gi].sbeg[s]; li < du.g[gi].sbeg[s + 1]; ++li) ust.wmask[vi] |= 1u << P.lits[li].tcol[L.kidx[1]];
+        }
+        ust.lists.push_back(L);
+        ust.sizes.push_back(tl->tuples);
+      }
+      ustreams.push_back(ust);
     }
     for (int s = 0; s < du.nsub; ++s) {
       du.sub[s].nk = (int)subsets[s].first.size();
       for (int k = 0; k < du.sub[s].nk; ++k) du.sub[s].pcol[k] = subsets[s].first[k];
-      if (!rp.heavy_q.empty() && !scalar) {
+      if (!rp.heavy_q.empty()) {
         auto ht = heavy_tables(ctx, rp, subsets[s].first);
         du.sub[s].hbuf = ht->buf.as<uint32_t>();
         du.sub[s].hbase = ht->base.as<int64_t>();
@@ -1776,9 +2131,10 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   // so each chunk of binding rows is staged once for all of them); otherwise one unit each.
   std::vector<std::vector<int>> members;
   if (o.mode == QF_EVAL_BATCHED) {
-    std::map<int, std::vector<int>> by_r;
-    for (int u = 0; u < (int)P.units.size(); ++u)
-      if (!P.unit_scalar[u]) by_r[P.units[u].R].push_back(u);
+    // Streamed units form batches of their own, which k_eval runs in the cells the streamed form
+    // leaves.
+    std::map<std::pair<int, int>, std::vector<int>> by_r;
+    for (int u = 0; u < (int)P.units.size(); ++u) by_r[std::make_pair(P.units[u].R, (int)P.unit_streamed[u])].push_back(u);
     for (auto& kv : by_r) {
       std::vector<int> cur;
       std::set<int> cur_vars;
@@ -1801,6 +2157,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   for (const auto& m : members) {
     DevBatch b{};
     b.R = P.units[m[0]].R;
+    b.heavy_only = P.unit_streamed[m[0]];
     b.shift = P.R_T - b.R;
     std::vector<int> bvars;
     for (int u : m)
@@ -1845,20 +2202,88 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     P.max_items = std::max(P.max_items, items);
     P.launch_items.push_back(items);
   }
-  if (!P.sunits.empty()) {
+  if (!ustreams.empty()) {
+    // Launches of at most TS_MAXL lists and TS_MAXT targets; every launch stages all join
+    // variables with the same rows per item, so that all agree on which cells are light.
     const int64_t cells = (int64_t)1 << P.R_T;
-    P.d_sunits = DevBuf(ctx, P.sunits.size() * sizeof(ScalarUnit));
-    copy_h2d(ctx, P.d_sunits.get(), P.sunits.data(), P.sunits.size() * sizeof(ScalarUnit));
+    const int nvars = (int)P.join_vars.size();
+    struct Pack {
+      std::vector<TsList> lists;
+      std::vector<int64_t> sizes;
+      std::vector<TsTarget> targets;
+      uint32_t wmask[EV_MAXG] = {};
+    };
+    std::vector<Pack> packs(1);
+    for (const UnitStream& u : ustreams) {
+      if (packs.back().lists.size() + u.lists.size() > (size_t)TS_MAXL ||
+          packs.back().targets.size() + u.targets.size() > (size_t)TS_MAXT)
+        packs.emplace_back();
+      Pack& pk = packs.back();
+      const int t0 = (int)pk.targets.size();
+      for (TsList L : u.lists) {
+        for (int v = 0; v < EV_MAXG; ++v)
+          if (L.target[v] >= 0) L.target[v] += t0;
+        pk.lists.push_back(L);
+      }
+      pk.sizes.insert(pk.sizes.end(), u.sizes.begin(), u.sizes.end());
+      pk.targets.insert(pk.targets.end(), u.targets.begin(), u.targets.end());
+      for (int v = 0; v < EV_MAXG; ++v) pk.wmask[v] |= u.wmask[v];
+    }
+    int tr = TS_ROWS / std::max(1, nvars);
+    for (const Pack& pk : packs) {
+      int pairs = 0;
+      for (int v = 0; v < nvars; ++v) pairs += __builtin_popcount(pk.wmask[v] & ~(1u << tidx(P.join_vars[v])));
+      if (pairs > 0) tr = std::min(tr, TS_PMAX / pairs);
+    }
+    QF_REQUIRE(tr >= 1, "too many paired binding columns for the streamed form");
+    int64_t bound = 0;
+    for (const Pack& pk : packs) {
+      StepPlan::StreamLaunch SL;
+      SL.d_lists = DevBuf(ctx, std::max<size_t>(1, pk.lists.size()) * sizeof(TsList));
+      if (!pk.lists.empty()) copy_h2d(ctx, SL.d_lists.get(), pk.lists.data(), pk.lists.size() * sizeof(TsList));
+      SL.d_targets = DevBuf(ctx, std::max<size_t>(1, pk.targets.size()) * sizeof(TsTarget));
+      if (!pk.targets.empty())
+        copy_h2d(ctx, SL.d_targets.get(), pk.targets.data(), pk.targets.size() * sizeof(TsTarget));
+      TsLaunch tl{};
+      tl.lists = SL.d_lists.as<TsList>();
+      tl.nlists = (int)pk.lists.size();
+      tl.targets = SL.d_targets.as<TsTarget>();
+      tl.ntargets = (int)pk.targets.size();
+      tl.nvars = nvars;
+      tl.ntc = (int)P.bcols.size();
+      tl.tr = tr;
+      for (int v = 0; v < nvars; ++v) {
+        tl.jt[v] = tidx(P.join_vars[v]);
+        tl.wmask[v] = pk.wmask[v];
+        for (size_t c = 0; c <= P.bcols.size(); ++c) tl.tcols[v][c] = P.tparts[v][c].as<int32_t>();
+        tl.toff[v] = P.toffs[v].as<int64_t>();
+      }
+      tl.cells = cells;
+      SL.d_launch = DevBuf(ctx, sizeof(TsLaunch));
+      copy_h2d(ctx, SL.d_launch.get(), &tl, sizeof(TsLaunch));
+      // Items: at most TS_CMAX binding chunks per cell times its parts, which number at most
+      // one per cell with entries plus the cell's wave chunks over TS_PART.
+      int64_t chunks = 0;
+      for (size_t i = 0; i < pk.lists.size(); ++i) chunks += ceil_div(pk.sizes[i], ts_chunk(pk.lists[i].m)) + cells;
+      SL.items_bound = (int64_t)TS_CMAX * (cells + chunks / TS_PART + 1);
+      bound = std::max(bound, SL.items_bound);
+      P.slaunches.push_back(std::move(SL));
+    }
     const int64_t cell_range[2] = {0, cells};
-    P.d_sc_cells = DevBuf(ctx, sizeof(cell_range));
-    copy_h2d(ctx, P.d_sc_cells.get(), cell_range, sizeof(cell_range));
-    P.d_sc_counts = DevBuf(ctx, cells * 8);
-    P.d_sc_starts = DevBuf(ctx, cells * 8);
-    // A cell's items are the chunks of its longest join variable: at most one more than the
-    // chunks of all its join variables' rows together.
-    P.sc_items = cells + ceil_div((int64_t)P.join_vars.size() * binding.num_rows, SC_ROWS) + 1;
-    P.d_sc_items = DevBuf(ctx, P.sc_items * sizeof(DevItem));
-    P.d_sc_num = DevBuf(ctx, 8);
+    P.d_ts_cells = DevBuf(ctx, sizeof(cell_range));
+    copy_h2d(ctx, P.d_ts_cells.get(), cell_range, sizeof(cell_range));
+    P.d_ts_counts = DevBuf(ctx, cells * 8);
+    P.d_ts_starts = DevBuf(ctx, cells * 8);
+    P.d_light = DevBuf(ctx, cells);
+    P.ts_items = bound;
+    P.d_ts_items = DevBuf(ctx, bound * sizeof(DevItem));
+    P.d_ts_num = DevBuf(ctx, 8);
+    P.ts_lds = (size_t)P.bcols.size() * TS_ROWS * 4 + (size_t)(2 * TS_VCAP + TS_PCAP) * 4 + TS_ROWS;
+    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_stream),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.ts_lds));
+    int per_cu = 0;
+    QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stream, TS_THREADS, P.ts_lds));
+    P.ts_blocks = ctx->num_cus * std::max(1, per_cu);
   }
   P.d_units = DevBuf(ctx, std::max<size_t>(1, P.units.size()) * sizeof(DevUnit));
   P.d_batches = DevBuf(ctx, std::max<size_t>(1, P.batches.size()) * sizeof(DevBatch));
@@ -1946,40 +2371,36 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
         }
       }
     }
-    // Single-key units, 32 at a time (one flag bit each).
-    for (size_t u0 = 0; u0 < P.sunits.size(); u0 += SC_UNITS) {
-      ScalarArgs sa{};
-      sa.units = P.d_sunits.as<ScalarUnit>() + u0;
-      sa.nunits = (int)std::min<size_t>(SC_UNITS, P.sunits.size() - u0);
-      sa.nvars = (int)P.join_vars.size();
-      for (int v = 0; v < sa.nvars; ++v) {
-        const int c = (int)(std::find(P.bcols.begin(), P.bcols.end(), P.join_vars[v]) - P.bcols.begin());
-        sa.vals[v] = P.tparts[v][c].as<int32_t>();
-        sa.tids[v] = P.tparts[v][P.bcols.size()].as<int32_t>();
-        sa.toff[v] = P.toffs[v].as<int64_t>();
-      }
-      sa.cells = (int64_t)1 << P.R_T;
-      sa.num_positive = (int)num_positive;
-      sa.lits = P.d_lits.as<DevLit>();
-      sa.items = P.d_sc_items.as<DevItem>();
-      sa.num_items = P.d_sc_num.as<int64_t>();
-      sa.counts_out = reinterpret_cast<unsigned long long*>(P.d_copies.get());
-      sa.copy_words = (int64_t)P.num_slots * 4;
-      sa.ncopies = P.ncopies;
+    // Streamed units, launch by launch; the first launch marks the light cells.
+    const int64_t cells_T = (int64_t)1 << P.R_T;
+    for (size_t li = 0; li < P.slaunches.size(); ++li) {
+      const StepPlan::StreamLaunch& SL = P.slaunches[li];
       {
         PhaseScope ps(ctx, "work_items");
-        hipLaunchKernelGGL(k_scalar_count, dim3((unsigned)ceil_div(sa.cells, 256)), dim3(256), 0, ctx->stream, sa,
-                           P.d_sc_counts.as<int64_t>());
+        hipLaunchKernelGGL(k_ts_count, dim3((unsigned)ceil_div(cells_T, 256)), dim3(256), 0, ctx->stream,
+                           SL.d_launch.as<TsLaunch>(), P.d_ts_counts.as<int64_t>(),
+                           li == 0 ? P.d_light.as<uint8_t>() : nullptr);
         QF_CHECK_HIP(hipGetLastError());
-        scan_exclusive_i64(ctx, P.d_sc_counts.as<int64_t>(), P.d_sc_starts.as<int64_t>(), sa.cells);
-        hipLaunchKernelGGL(k_expand_items, dim3((unsigned)ceil_div(P.sc_items, 256)), dim3(256), 0, ctx->stream,
-                           P.d_sc_cells.as<int64_t>(), 1, P.d_sc_counts.as<int64_t>(), P.d_sc_starts.as<int64_t>(),
-                           P.d_sc_items.as<DevItem>(), P.d_sc_num.as<int64_t>());
+        scan_exclusive_i64(ctx, P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), cells_T);
+        hipLaunchKernelGGL(k_expand_items, dim3((unsigned)ceil_div(SL.items_bound, 256)), dim3(256), 0, ctx->stream,
+                           P.d_ts_cells.as<int64_t>(), 1, P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(),
+                           P.d_ts_items.as<DevItem>(), P.d_ts_num.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
       }
-      PhaseScope ps(ctx, "probe_count_single_key");
-      const int blocks = (int)std::min<int64_t>((int64_t)ctx->num_cus * 4, P.sc_items);
-      hipLaunchKernelGGL(k_scalar_units, dim3((unsigned)blocks), dim3(SC_THREADS), 0, ctx->stream, sa);
+      TsArgs ta{};
+      ta.launch = SL.d_launch.as<TsLaunch>();
+      ta.units = P.d_units.as<DevUnit>();
+      ta.lits = P.d_lits.as<DevLit>();
+      ta.items = P.d_ts_items.as<DevItem>();
+      ta.num_items = P.d_ts_num.as<int64_t>();
+      ta.max_items = P.ts_items;
+      ta.num_positive = (int)num_positive;
+      ta.counts_out = reinterpret_cast<unsigned long long*>(P.d_copies.get());
+      ta.copy_words = (int64_t)P.num_slots * 4;
+      ta.ncopies = P.ncopies;
+      PhaseScope ps(ctx, "probe_count_stream");
+      const int blocks = (int)std::min<int64_t>(P.ts_blocks, std::max<int64_t>(1, SL.items_bound));
+      hipLaunchKernelGGL(k_stream, dim3((unsigned)blocks), dim3(TS_THREADS), P.ts_lds, ctx->stream, ta);
       QF_CHECK_HIP(hipGetLastError());
     }
     EvalArgs a{};
@@ -2011,7 +2432,7 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
         PhaseScope ps(ctx, "work_items");
         const dim3 grid((unsigned)ceil_div(cells, 256));
         hipLaunchKernelGGL(k_count_items, grid, dim3(256), 0, ctx->stream, a.batches, a.units, unit_cells, nu, P.tile,
-                           P.d_cell_counts.as<int64_t>());
+                           P.d_light.as<uint8_t>(), P.d_cell_counts.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_cell_counts.as<int64_t>(), P.d_cell_starts.as<int64_t>(), cells);
         hipLaunchKernelGGL(k_expand_items, dim3((unsigned)std::max<int64_t>(1, ceil_div(items_bound, 256))), dim3(256),

@@ -38,21 +38,6 @@ def test_probe_bytes_per_join_group():
     assert total == want
 
 
-def test_probe_bytes_split_between_the_two_kernels():
-    # Unit (relation 0, position 0) has literals binding two positions, as has unit (1, 1); the
-    # added literal (1, [-1, -1, 1]) makes unit (1, 2), whose only literal binds one position.
-    w = _workload()
-    w["literals"] = w["literals"] + [(1, [-1, -1, 1])]
-    single = bench.single_key_units(w["literals"])
-    assert single == {(1, 2)}
-    total, groups = bench.algorithmic_probe_bytes(w)
-    rest, rest_groups = bench.algorithmic_probe_bytes(w, single, exclude=True)
-    one, one_groups = bench.algorithmic_probe_bytes(w, single)
-    assert (rest_groups, one_groups, groups) == (3, 1, 4)
-    assert one == 8 * 70 + 8 * 1000 + 32 * 1
-    assert rest + one == total
-
-
 def test_partition_bytes_per_join_variable():
     # Join variables {X0, X2}; referenced binding columns {X0, X1, X2}: each variable's partition
     # reads 3 columns and writes 3 plus the row id.
