@@ -364,7 +364,6 @@ __global__ void k_count_items(const DevBatch* batches, const DevUnit* units, con
 // Cells whose binding rows exceed TS_CMAX chunks are left to k_eval.
 
 constexpr int TS_THREADS = 512;
-constexpr int TS_WAVES = TS_THREADS / 64;
 constexpr int TS_ROWS = 1024;          // staged binding rows per work item (all join variables)
 constexpr int TS_VCAP = 2048;          // value-table slots (at most TS_ROWS entries: load <= 1/2)
 constexpr int TS_VLOG = 11;
@@ -375,23 +374,27 @@ constexpr int TS_MAXL = 128;           // entry lists per launch
 constexpr int TS_MAXT = 256;           // (unit, join variable) counters of join-column literals per launch
 constexpr int TS_PART = 512;           // wave chunks of relation entries per work item
 constexpr int TS_CMAX = 4;             // binding chunks of a cell that the streamed form takes
-constexpr int TS_RUNCH = 16;           // runs per wave chunk: 16-lane rows, one join variable each
+constexpr int TS_RUNROW = 16;          // runs per row of lanes (a row's runs belong to one list)
+constexpr int TS_RUNROWS = 4;          // rows per wave chunk
 constexpr int TS_TUPCH = 256;          // tuples per wave chunk (four per lane)
-constexpr int TS_STAGE = 8;            // binding words a thread stages per round (loads in flight)
 static_assert(TS_VCAP == (1 << TS_VLOG) && TS_PCAP == (1 << TS_PLOG), "table sizes are powers of two");
 static_assert(TS_ROWS <= 2047 && TS_ROWS <= 0xffff && EV_MAXTC <= 16 && EV_MAXG <= 16 && 2 * EV_MAXG < 256,
               "slot words hold rows, columns and variables; row flags hold the variable");
 static_assert(TS_VCAP >= 2 * TS_ROWS, "value table at load <= 1/2");
+static_assert(TS_ROWS % TS_THREADS == 0, "staged rows are split evenly over the threads");
 
 // Distinct key tuples of a (relation, join column) partition for one key subset, in cell order.
 struct TsList {
   const int32_t* col[EV_MAXK];  // col[0]: join column value; col[1]: the pair partner; then the rest
-  const int32_t* cnt;           // facts with the tuple
+  const int32_t* cnt;           // facts with the tuple; bit 31 (m >= 3): the first two columns differ
+                                // from the previous tuple's
   const int64_t* off;           // per cell: first entry (cells + 1)
   int m;                        // key columns (1: runs of the join column)
   int unit;                     // DevUnit of the literal ranges (m >= 2)
   int subset;                   // the unit's subset (m >= 2)
   int kidx[EV_MAXK];            // list column i is key position kidx[i] of the subset's literals
+  int psubset;                  // m >= 3: the unit's subset of the first two columns, counted here too, or -1
+  int pkidx1;                   // its key position of list column 1
   int target[EV_MAXG];          // m == 1: per join variable, its counter or -1
   int vgroup[EV_MAXG];          // m >= 2: per join variable, the unit's group or -1
 };
@@ -401,8 +404,9 @@ struct TsTarget {
 };
 
 struct TsLaunch {
-  const TsList* lists;
+  const TsList* lists;  // run lists (m == 1) first, then tuple lists
   int nlists;
+  int nrun;
   const TsTarget* targets;
   int ntargets;
   int nvars;   // join variables (staged segments)
@@ -415,11 +419,13 @@ struct TsLaunch {
   int64_t cells;
 };
 
-__host__ __device__ inline int ts_chunk(int m) { return m == 1 ? TS_RUNCH : TS_TUPCH; }
 
-// Work items per cell: (binding chunks) x (parts of the cell's entries); light[q] = 1 when the
-// cell's binding rows fit TS_CMAX chunks (k_eval then skips it).
-__global__ void k_ts_count(const TsLaunch* L_, int64_t* counts, uint8_t* light) {
+// Work items per cell: (binding chunks) x (parts of the cell's wave chunks: its runs, all run
+// lists one after another in rows of TS_RUNROW, TS_RUNROWS rows per chunk, then its tuples in
+// chunks of TS_TUPCH); light[q] = 1 when the cell's binding rows fit TS_CMAX chunks (k_eval then
+// skips it).  cellmat[q * nlists + l] = (first entry, entries) of list l in cell q, so that a work
+// item reads its cell's lists in one pass.
+__global__ void k_ts_count(const TsLaunch* L_, int64_t* counts, int32_t* parts, int2* cellmat, uint8_t* light) {
   cptr<TsLaunch> L = as_const(L_);
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= L->cells) return;
@@ -430,30 +436,59 @@ __global__ void k_ts_count(const TsLaunch* L_, int64_t* counts, uint8_t* light) 
   }
   const bool lt = chunks <= TS_CMAX;
   if (light) light[q] = lt ? 1 : 0;
-  int64_t wc = 0;
-  if (lt && chunks > 0) {
-    cptr<TsList> lists = as_const(L->lists);
-    for (int l = 0; l < L->nlists; ++l) {
-      const int64_t n = lists[l].off[q + 1] - lists[l].off[q];
-      const int csz = ts_chunk(lists[l].m);
-      wc += (n + csz - 1) / csz;
-    }
+  cptr<TsList> lists = as_const(L->lists);
+  int64_t rows = 0, nt = 0;
+  for (int l = 0; l < L->nlists; ++l) {
+    const int64_t b = lists[l].off[q], n = lists[l].off[q + 1] - b;
+    cellmat[q * L->nlists + l] = make_int2((int)b, (int)n);
+    if (l < L->nrun) rows += (n + TS_RUNROW - 1) / TS_RUNROW;
+    else nt += n;
   }
-  counts[q] = chunks * ((wc + TS_PART - 1) / TS_PART);
+  const int64_t wc = (rows + TS_RUNROWS - 1) / TS_RUNROWS + (nt + TS_TUPCH - 1) / TS_TUPCH;
+  const int64_t p = (wc + TS_PART - 1) / TS_PART;
+  parts[q] = (int32_t)max<int64_t>(1, p);
+  counts[q] = (lt && chunks > 0) ? chunks * p : 0;
+}
+
+// One thread per work item: item i of cell q (the last cell whose scanned start is <= i) is
+// binding chunk (i - start) / parts[q], part (i - start) % parts[q].
+__global__ void k_ts_expand(const int64_t* counts, const int64_t* starts, const int32_t* parts, int64_t cells,
+                            DevItem* items, int64_t* total) {
+  const int64_t all = starts[cells - 1] + counts[cells - 1];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *total = all;
+  if (i >= all) return;
+  int64_t lo = 0, hi = cells - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const int idx = (int)(i - starts[lo]), p = parts[lo];
+  items[i] = DevItem{idx % p, idx / p, lo};
 }
 
 __device__ inline uint32_t ts_vhash(int32_t x, int v) { return ((uint32_t)x * 0x9e3779b1u + (uint32_t)v) * 0x85ebca6bu; }
 __device__ inline uint32_t ts_phash(int32_t x, int32_t y) {
   return (((uint32_t)x * 0x9e3779b1u) ^ (uint32_t)y) * 0x85ebca6bu;
 }
-// Value slot word: fingerprint (17 bits) << 15 | join variable << 11 | (1 + representative row).
+// Value slot (64 bits): the join value in the low word; above it the value's negative rows (bits
+// 32-44), positive rows (45-57) and join variable (58-61) of the item, and bit 63 set once the
+// slot is taken (0 = empty).  A probe compares value and variable in one read.
 // Pair slot word: fingerprint (13 bits) << 19 | join variable << 15 | partner column << 11 |
 // (1 + row).  0 = empty.
-constexpr uint32_t TS_VKEY_MASK = ~0x7ffu;
+constexpr unsigned long long TS_VTAKEN = 1ull << 63;
+constexpr unsigned long long TS_VKEYBITS = 0xffffffffull | (0xfull << 58) | TS_VTAKEN;
+constexpr int TS_VNEG = 32, TS_VPOS = 45, TS_VVAR = 58;
+static_assert(TS_ROWS < (1 << 13), "a value's rows fit 13 bits");
+__device__ inline unsigned long long ts_vkey(int32_t x, int vj) {
+  return TS_VTAKEN | ((unsigned long long)vj << TS_VVAR) | (uint32_t)x;
+}
 constexpr uint32_t TS_PFP_MASK = ~((1u << 19) - 1u);
 
 struct TsArgs {
   const TsLaunch* launch;
+  const int2* cellmat;  // per cell and list: first entry, entries
   const DevUnit* units;
   const DevLit* lits;
   const DevItem* items;
@@ -474,18 +509,31 @@ __device__ __attribute__((noinline)) void ts_tuple_match(const TsArgs& a, cptr<T
   const int g = D->vgroup[vj];
   if (g < 0) return;
   cptr<DevUnit> U = as_const(a.units) + D->unit;
-  const int l0 = U->g[g].sbeg[D->subset], l1 = U->g[g].sbeg[D->subset + 1];
   cptr<DevLit> lits = as_const(a.lits);
   const int m = D->m;
+  const uint32_t cw = (uint32_t)D->cnt[pos];
+  const unsigned long long facts = cw & 0x7fffffffu;
+  const bool p = (rflag[r] & 1) != 0;
+  if (D->psubset >= 0) {
+    // The first two columns' subset: every tuple of a (column 0, column 1) group adds its facts,
+    // the group's first tuple the covered row.
+    const int l0 = U->g[g].sbeg[D->psubset], l1 = U->g[g].sbeg[D->psubset + 1];
+    for (int li = l0; li < l1; ++li) {
+      if (lits[li].tcol[D->pkidx1] != w) continue;
+      unsigned long long* c = copy + (int64_t)lits[li].slot * 4;
+      if (cw >> 31) atomicAdd(c + (p ? 0 : 1), 1ull);
+      atomicAdd(c + (p ? 2 : 3), facts);
+    }
+  }
+  const int l0 = U->g[g].sbeg[D->subset], l1 = U->g[g].sbeg[D->subset + 1];
   for (int li = l0; li < l1; ++li) {
     if (lits[li].tcol[D->kidx[1]] != w) continue;
     bool ok = true;
     for (int i = 2; i < m && ok; ++i) ok = D->col[i][pos] == tstage[lits[li].tcol[D->kidx[i]] * TS_ROWS + r];
     if (!ok) continue;
-    const bool p = (rflag[r] & 1) != 0;
     unsigned long long* c = copy + (int64_t)lits[li].slot * 4;
     atomicAdd(c + (p ? 0 : 1), 1ull);
-    atomicAdd(c + (p ? 2 : 3), (unsigned long long)(uint32_t)D->cnt[pos]);
+    atomicAdd(c + (p ? 2 : 3), facts);
   }
 }
 
@@ -514,59 +562,97 @@ __device__ __attribute__((noinline)) void ts_equal_pair(const TsArgs& a, cptr<Ts
   }
 }
 
-// Probe of the pair table for tuple (x, y) at list position pos; every matching entry is
-// handed to ts_tuple_match.
-__device__ inline void ts_pair_probe(const TsArgs& a, cptr<TsList> D, int64_t pos, int32_t x, int32_t y,
-                                     const uint32_t* pword, const int32_t* tstage, const uint8_t* rflag, const int* jt,
-                                     unsigned long long* copy) {
-  const uint32_t h = ts_phash(x, y);
-  const uint32_t fp = ((h >> 4) & 0x1fffu) << 19;
-  uint32_t slot = h >> (32 - TS_PLOG);
-  while (true) {
-    const uint32_t wd = pword[slot];
-    if (wd == 0) break;
-    if ((wd & TS_PFP_MASK) == fp) {
-      const int r = (int)(wd & 0x7ffu) - 1, w = (int)((wd >> 11) & 15u), vj = (int)((wd >> 15) & 15u);
-      if (tstage[jt[vj] * TS_ROWS + r] == x && tstage[w * TS_ROWS + r] == y)
-        ts_tuple_match(a, D, pos, r, vj, w, tstage, rflag, copy);
+// Probes of the pair table for up to TS_TJ tuples of a lane, interleaved (their slot reads in
+// flight together); every matching entry is handed to ts_tuple_match.
+constexpr int TS_TJ = TS_TUPCH / 64;
+__device__ inline void ts_pair_probes(const TsArgs& a, cptr<TsList> lists, const int (&lj)[TS_TJ],
+                                      const int64_t (&pos)[TS_TJ], const int32_t (&x)[TS_TJ],
+                                      const int32_t (&y)[TS_TJ], const uint32_t* pword, const int32_t* tstage,
+                                      const uint8_t* rflag, const int* jt, unsigned long long* copy) {
+  uint32_t slot[TS_TJ], fp[TS_TJ];
+  bool live[TS_TJ];
+#pragma unroll
+  for (int j = 0; j < TS_TJ; ++j) {
+    const uint32_t h = ts_phash(x[j], y[j]);
+    fp[j] = ((h >> 4) & 0x1fffu) << 19;
+    slot[j] = h >> (32 - TS_PLOG);
+    live[j] = pos[j] >= 0;
+  }
+  // The table is read four slots (one aligned 16-byte group) at a time: a probe checks the slots
+  // from its current one to the group's end and stops at the first empty slot.  The common case
+  // (no fingerprint hit) is branch-free: bit masks of the group's empty slots and fingerprint
+  // hits; only a hit goes to the key check.
+  const uint4* groups = reinterpret_cast<const uint4*>(pword);
+  while (live[0] || live[1] || live[2] || live[3]) {
+    uint4 g[TS_TJ];
+#pragma unroll
+    for (int j = 0; j < TS_TJ; ++j) g[j] = live[j] ? groups[slot[j] >> 2] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int j = 0; j < TS_TJ; ++j) {
+      const uint32_t wds[4] = {g[j].x, g[j].y, g[j].z, g[j].w};
+      uint32_t empty = 0, hit = 0;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        empty |= (wds[s] == 0u ? 1u : 0u) << s;
+        hit |= ((wds[s] & TS_PFP_MASK) == fp[j] ? 1u : 0u) << s;
+      }
+      const uint32_t from = 0xfu << (slot[j] & 3u);       // slots at or after the current one
+      const uint32_t stop = empty & from;                  // the probe ends at the first of these
+      const uint32_t before = stop ? (stop & (0u - stop)) - 1u : 0xfu;  // slots before that end
+      uint32_t cand = live[j] ? hit & from & before & ~empty : 0u;
+      while (cand) {
+        const int s = __ffs((int)cand) - 1;
+        cand &= cand - 1;
+        const uint32_t wd = wds[s];
+        const int r = (int)(wd & 0x7ffu) - 1, w = (int)((wd >> 11) & 15u), vj = (int)((wd >> 15) & 15u);
+        if (tstage[jt[vj] * TS_ROWS + r] == x[j] && tstage[w * TS_ROWS + r] == y[j])
+          ts_tuple_match(a, lists + lj[j], pos[j], r, vj, w, tstage, rflag, copy);
+      }
+      live[j] = live[j] && stop == 0u;
+      slot[j] = ((slot[j] | 3u) + 1u) & (TS_PCAP - 1);
     }
-    slot = (slot + 1) & (TS_PCAP - 1);
   }
 }
+static_assert(TS_TJ == 4, "the probe loop tests four tuples");
 
-__global__ __launch_bounds__(TS_THREADS) void k_stream(TsArgs a) {
+// Four waves per SIMD: two 512-thread workgroups per CU, as LDS allows.
+__global__ __launch_bounds__(TS_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_stream(TsArgs a) {
   extern __shared__ uint32_t lds[];
   cptr<TsLaunch> L = as_const(a.launch);
   const int ntc = L->ntc;
   int32_t* tstage = reinterpret_cast<int32_t*>(lds);   // ntc x TS_ROWS
-  uint32_t* vword = lds + ntc * TS_ROWS;               // TS_VCAP
-  uint32_t* vpn = vword + TS_VCAP;                     // TS_VCAP: positive rows << 16 | negative rows
-  uint32_t* pword = vpn + TS_VCAP;                     // TS_PCAP
+  unsigned long long* vtab = reinterpret_cast<unsigned long long*>(lds + ntc * TS_ROWS);  // TS_VCAP
+  uint32_t* pword = lds + ntc * TS_ROWS + 2 * TS_VCAP;                                       // TS_PCAP
   uint8_t* rflag = reinterpret_cast<uint8_t*>(pword + TS_PCAP);  // TS_ROWS: join variable << 1 | positive
   __shared__ unsigned long long acc[TS_MAXT][4];
-  __shared__ const int64_t* loff[TS_MAXL];  // per list: its cell offsets
-  __shared__ int lm[TS_MAXL];
-  __shared__ int64_t lbase[TS_MAXL];        // per list: the item's cell: first entry, entries, first wave chunk
-  __shared__ int lcnt[TS_MAXL];
-  __shared__ int lch[TS_MAXL + 1];
+  __shared__ const int32_t* lcol0[TS_MAXL];  // per list: join values (runs) or first key column (tuples)
+  __shared__ const int32_t* lcol1[TS_MAXL];  // counts (runs) or second key column (tuples)
+  __shared__ int lbase[TS_MAXL];             // per list, in the item's cell: first entry (a list holds
+                                             // fewer than 2^31 entries),
+  __shared__ int lcnt[TS_MAXL];              // entries,
+  __shared__ int lstart[TS_MAXL + 1];        // first row (run lists) or first position (tuple lists)
+  __shared__ int16_t ltgt[TS_MAXL][4];       // per run list: the counters of join variables 0-3
+  __shared__ uint8_t cmap[TS_PART + 1];      // per wave chunk of the item (and the next): its first list
   __shared__ const int32_t* tcol[EV_MAXG][EV_MAXTC + 1];
   __shared__ const int64_t* toff[EV_MAXG];
   __shared__ int seg_off[EV_MAXG + 1];
-  __shared__ int64_t seg_src[EV_MAXG];
   __shared__ int jt[EV_MAXG];
-  __shared__ int item_part;
+  __shared__ int item_rows, item_tuples, next_chunk;
+  static_assert(TS_MAXT < 32768 && TS_MAXL <= 256, "target entries and list indices fit their types");
 
   const int lane = threadIdx.x & 63;
   const int wave = uni(threadIdx.x >> 6);
-  const int nvars = L->nvars, nlists = L->nlists;
+  const int nvars = L->nvars, nlists = L->nlists, nrun = L->nrun;
   const int64_t total = min(uni64(*a.num_items), a.max_items);
   cptr<DevItem> items = as_const(a.items);
   cptr<TsList> lists = as_const(L->lists);
   unsigned long long* copy = a.counts_out + (int64_t)(blockIdx.x % a.ncopies) * a.copy_words;
   for (int i = threadIdx.x; i < TS_MAXT * 4; i += TS_THREADS) (&acc[0][0])[i] = 0;
   for (int i = threadIdx.x; i < nlists; i += TS_THREADS) {
-    loff[i] = lists[i].off;
-    lm[i] = lists[i].m;
+    lcol0[i] = lists[i].col[0];
+    lcol1[i] = lists[i].m == 1 ? lists[i].cnt : lists[i].col[1];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ltgt[i][v] = (int16_t)(v < nvars ? lists[i].target[v] : -1);
   }
   for (int i = threadIdx.x; i < nvars * (ntc + 1); i += TS_THREADS) tcol[i / (ntc + 1)][i % (ntc + 1)] = L->tcols[i / (ntc + 1)][i % (ntc + 1)];
   if ((int)threadIdx.x < nvars) {
@@ -576,129 +662,164 @@ __global__ __launch_bounds__(TS_THREADS) void k_stream(TsArgs a) {
 
   for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
     const int64_t q = items[it].q;
-    const int idx = items[it].chunk;  // the item's index within its cell
+    const int chunk = items[it].chunk, part = items[it].batch;  // binding chunk, part of the cell's entries
     __syncthreads();  // the previous item is done with the tables, rows and lists
+    // Binding rows of the chunk: the join variables' segments one after another (wave-uniform
+    // loop); each thread notes the segment and source row of its two staged rows.
+    constexpr int RPT = TS_ROWS / TS_THREADS;  // staged rows per thread
+    int rv[RPT];
+    int64_t rsrc[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      rv[j] = -1;
+      rsrc[j] = 0;
+    }
+    int rows_all = 0;
+    for (int v = 0; v < nvars; ++v) {
+      const int64_t t0 = L->toff[v][q], t1 = L->toff[v][q + 1];
+      const int64_t s = t0 + (int64_t)chunk * L->tr;
+      const int n = (int)max<int64_t>(0, min<int64_t>(L->tr, t1 - s));
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int r = (int)threadIdx.x + j * TS_THREADS;
+        if (r >= rows_all && r < rows_all + n) {
+          rv[j] = v;
+          rsrc[j] = s + (r - rows_all);
+        }
+      }
+      if (threadIdx.x == 0) seg_off[v] = rows_all;
+      rows_all += n;
+    }
+    if (threadIdx.x == 0) seg_off[nvars] = rows_all;
     if (wave == 0) {
-      // Entry lists of the cell (two per lane, loads in flight together with the join
-      // variables' cell bounds): first entry, entries, first wave chunk.
-      int64_t b0[2] = {0, 0}, b1[2] = {0, 0};
+      // The cell's entries of every list (two lists per lane): runs in rows of TS_RUNROW (each
+      // list's runs start a row), tuples one after another.
+      int2 bn[2] = {make_int2(0, 0), make_int2(0, 0)};
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int l = lane + 64 * j;
-        if (l < nlists) {
-          b0[j] = loff[l][q];
-          b1[j] = loff[l][q + 1];
-        }
+        if (l < nlists) bn[j] = a.cellmat[q * nlists + l];
       }
-      int64_t t0 = 0, t1 = 0;
-      if (lane < nvars) {
-        t0 = toff[lane][q];
-        t1 = toff[lane][q + 1];
-      }
-      int carry = 0;
+      int rows = 0, tuples = 0;
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int l = lane + 64 * j;
-        int ch = 0;
-        if (l < nlists) {
-          const int n = (int)(b1[j] - b0[j]);
-          const int csz = ts_chunk(lm[l]);
-          ch = (n + csz - 1) / csz;
-          lbase[l] = b0[j];
-          lcnt[l] = n;
-        }
-        int x = ch;
+        const int n = l < nlists ? bn[j].y : 0;
+        const int nrw = l < nrun ? (n + TS_RUNROW - 1) / TS_RUNROW : 0;
+        const int ntp = l >= nrun ? n : 0;
+        int xr = nrw, xt = ntp;
         for (int off = 1; off < 64; off <<= 1) {
-          const int y = __shfl_up(x, off, 64);
-          if (lane >= off) x += y;
+          const int yr = __shfl_up(xr, off, 64), yt = __shfl_up(xt, off, 64);
+          if (lane >= off) {
+            xr += yr;
+            xt += yt;
+          }
         }
-        if (l < nlists) lch[l] = carry + x - ch;
-        carry += __shfl(x, 63, 64);
+        if (l < nlists) {
+          lbase[l] = bn[j].x;
+          lcnt[l] = n;
+          lstart[l] = l < nrun ? rows + xr - nrw : tuples + xt - ntp;
+        }
+        rows += __shfl(xr, 63, 64);
+        tuples += __shfl(xt, 63, 64);
       }
-      const int parts = (carry + TS_PART - 1) / TS_PART;  // >= 1: the cell has items
-      const int chunk = idx / parts;
-      // Binding rows of the chunk, one lane per join variable.
-      int n = 0;
-      int64_t s0 = 0;
-      if (lane < nvars) {
-        s0 = t0 + (int64_t)chunk * L->tr;
-        n = (int)max<int64_t>(0, min<int64_t>(L->tr, t1 - s0));
-      }
-      int x = n;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-      }
-      if (lane < nvars) {
-        seg_off[lane] = x - n;
-        seg_src[lane] = s0;
-      }
-      if (lane == nvars - 1) seg_off[nvars] = x;
       if (lane == 0) {
-        lch[nlists] = carry;
-        item_part = idx - chunk * parts;
+        lstart[nlists] = tuples;
+        item_rows = rows;
+        item_tuples = tuples;
+        next_chunk = part * TS_PART;
       }
-    } else {
-      // The tables start empty.
+    }
+    // Stage the rows, two per thread (all columns of a row in flight together): the ntc binding
+    // columns and a flag (join variable << 1 | positive, from the row id); then empty the tables.
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int r = (int)threadIdx.x + j * TS_THREADS;
+      if (rv[j] < 0) continue;
+      const int v = rv[j];
+      const int64_t sr = rsrc[j];
+      // Columns in groups of five (loads in flight together); column ntc is the row id.
+      for (int cb = 0; cb <= ntc; cb += 5) {
+        int32_t val[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+          if (cb + k <= ntc) val[k] = tcol[v][cb + k][sr];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const int c = cb + k;
+          if (c < ntc) tstage[c * TS_ROWS + r] = val[k];
+          else if (c == ntc) rflag[r] = (uint8_t)((v << 1) | (val[k] < a.num_positive ? 1 : 0));
+        }
+      }
+    }
+    {
       const int n16 = (2 * TS_VCAP + TS_PCAP) / 4;
-      uint4* t = reinterpret_cast<uint4*>(vword);
-      for (int i = threadIdx.x - 64; i < n16; i += TS_THREADS - 64) t[i] = make_uint4(0u, 0u, 0u, 0u);
+      uint4* t = reinterpret_cast<uint4*>(vtab);
+      for (int i = threadIdx.x; i < n16; i += TS_THREADS) t[i] = make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
-    // Stage the rows: the ntc binding columns and a flag (join variable << 1 | positive, from
-    // the row id), segment after segment, column-major within a segment; each thread keeps
-    // TS_STAGE loads in flight.
-    const int rows_all = seg_off[nvars];
-    const int words_all = (ntc + 1) * rows_all;
-    for (int w0 = 0; w0 < words_all; w0 += TS_STAGE * TS_THREADS) {
-      int32_t val[TS_STAGE];
-      int dst[TS_STAGE];
-#pragma unroll
-      for (int j = 0; j < TS_STAGE; ++j) {
-        const int w = w0 + (int)threadIdx.x + j * TS_THREADS;
-        dst[j] = -1;
-        val[j] = 0;
-        if (w < words_all) {
-          int v = 0;
-          while (w >= (ntc + 1) * seg_off[v + 1]) ++v;
-          const int r0 = seg_off[v], n = seg_off[v + 1] - r0;
-          const int wi = w - (ntc + 1) * r0;
-          const int c = wi / n, i = wi - c * n;
-          val[j] = tcol[v][c][seg_src[v] + i];
-          dst[j] = c < ntc ? c * TS_ROWS + r0 + i : -2 - ((r0 + i) | (v << 16));
-        }
+    const int nrows = item_rows, nt = item_tuples;
+    const int rc = (nrows + TS_RUNROWS - 1) / TS_RUNROWS;
+    const int c_all = rc + (nt + TS_TUPCH - 1) / TS_TUPCH;
+    const int c0 = part * TS_PART, c1 = min(c0 + TS_PART, c_all);
+    // The first list of each of the item's wave chunks and of the chunk after them.
+    for (int t = threadIdx.x; t <= c1 - c0; t += TS_THREADS) {
+      const int k = c0 + t;
+      int lo, hi, key;
+      if (k < rc) {
+        lo = 0, hi = nrun - 1, key = k * TS_RUNROWS;
+      } else {
+        lo = nrun, hi = nlists - 1, key = (k - rc) * TS_TUPCH;
       }
-#pragma unroll
-      for (int j = 0; j < TS_STAGE; ++j) {
-        if (dst[j] >= 0) {
-          tstage[dst[j]] = val[j];
-        } else if (dst[j] <= -2) {
-          const int e = -2 - dst[j];
-          rflag[e & 0xffff] = (uint8_t)(((e >> 16) << 1) | (val[j] < a.num_positive ? 1 : 0));
-        }
+      if (k >= c_all) lo = hi = nlists - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (lstart[mid] <= key) lo = mid;
+        else hi = mid - 1;
       }
+      cmap[t] = (uint8_t)lo;
     }
-    __syncthreads();
-    // Build both tables.
+    // Build both tables.  A row's first value-table CAS and the CASes of up to four pair entries
+    // are in flight together; collisions then walk on one slot at a time.
     for (int r = threadIdx.x; r < rows_all; r += TS_THREADS) {
       const int fl = rflag[r];
       const int vj = fl >> 1;
       const int jc = jt[vj];
       const int32_t x = tstage[jc * TS_ROWS + r];
-      {
-        const uint32_t h = ts_vhash(x, vj);
-        const uint32_t key = (((h >> 4) & 0x1ffffu) << 15) | ((uint32_t)vj << 11);
-        uint32_t slot = h >> (32 - TS_VLOG);
-        while (true) {
-          const uint32_t prev = atomicCAS(&vword[slot], 0u, key | (uint32_t)(r + 1));
-          if (prev == 0) break;
-          if ((prev & TS_VKEY_MASK) == key && tstage[jc * TS_ROWS + (int)(prev & 0x7ffu) - 1] == x) break;
-          slot = (slot + 1) & (TS_VCAP - 1);
-        }
-        atomicAdd(&vpn[slot], (fl & 1) ? 0x10000u : 1u);
-      }
+      const unsigned long long vkey = ts_vkey(x, vj);
+      const unsigned long long vcount = 1ull << ((fl & 1) ? TS_VPOS : TS_VNEG);
+      uint32_t vslot = ts_vhash(x, vj) >> (32 - TS_VLOG);
       uint32_t wm = L->wmask[vj] & ~(1u << jc);  // pairs of the join variable with itself: see ts_equal_pair
-      while (wm) {
+      uint32_t ps[4], pw[4], pp[4];
+      int np = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        ps[k] = pw[k] = 0;
+        if (wm) {
+          const int w = __ffs((int)wm) - 1;
+          wm &= wm - 1;
+          const uint32_t h = ts_phash(x, tstage[w * TS_ROWS + r]);
+          pw[k] = (((h >> 4) & 0x1fffu) << 19) | ((uint32_t)vj << 15) | ((uint32_t)w << 11) | (uint32_t)(r + 1);
+          ps[k] = h >> (32 - TS_PLOG);
+          np = k + 1;
+        }
+      }
+      // A new value claims its slot with its first row counted; a repeated one adds its row.
+      unsigned long long vprev = atomicCAS(&vtab[vslot], 0ull, vkey | vcount);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pp[k] = k < np ? atomicCAS(&pword[ps[k]], 0u, pw[k]) : 0u;
+      while (vprev != 0 && (vprev & TS_VKEYBITS) != vkey) {
+        vslot = (vslot + 1) & (TS_VCAP - 1);
+        vprev = atomicCAS(&vtab[vslot], 0ull, vkey | vcount);
+      }
+      if (vprev != 0) atomicAdd(&vtab[vslot], vcount);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        while (pp[k] != 0) {
+          ps[k] = (ps[k] + 1) & (TS_PCAP - 1);
+          pp[k] = atomicCAS(&pword[ps[k]], 0u, pw[k]);
+        }
+      while (wm) {  // partners beyond the first four
         const int w = __ffs((int)wm) - 1;
         wm &= wm - 1;
         const uint32_t h = ts_phash(x, tstage[w * TS_ROWS + r]);
@@ -708,82 +829,129 @@ __global__ __launch_bounds__(TS_THREADS) void k_stream(TsArgs a) {
       }
     }
     __syncthreads();
-    // Stream the item's wave chunks of the cell's entries.
-    const int c_end = lch[nlists];
-    const int c0 = item_part * TS_PART, c1 = min(c0 + TS_PART, c_end);
-    int l = 0;
-    for (int k = c0 + wave; k < c1; k += TS_WAVES) {
-      while (lch[l + 1] <= k) ++l;  // lists with no chunk in this cell are passed over
-      l = uni(l);
-      cptr<TsList> D = lists + l;
-      const int n = lcnt[l];
-      const int64_t base = lbase[l];
-      if (lm[l] == 1) {
-        // Runs: lanes (join variable vg + lane / 16, run e + lane % 16).
-        const int e = (k - lch[l]) * TS_RUNCH + (lane & 15);
-        const bool in = e < n;
+    // Stream the item's wave chunks, each wave taking the next one as it finishes: run chunks of
+    // TS_RUNROWS rows of TS_RUNROW lanes (a row holds runs of one list), then tuple chunks of
+    // TS_TUPCH tuples (TS_TJ consecutive ones per lane).  A lane finds its list by binary search
+    // between the chunk's first list and the next chunk's.
+    while (true) {
+      int k = 0;
+      if (lane == 0) k = atomicAdd(&next_chunk, 1);
+      k = __shfl(k, 0, 64);
+      if (k >= c1) break;
+      const bool run_chunk = k < rc;
+      int lo = cmap[k - c0];
+      int hi = (k + 1 < c_all && (k + 1 < rc) == run_chunk) ? (int)cmap[k + 1 - c0] : run_chunk ? nrun - 1 : nlists - 1;
+      if (run_chunk) {
+        const int row = k * TS_RUNROWS + (lane >> 4);
+        const bool rin = row < nrows;
+        if (rin)
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (lstart[mid] <= row) lo = mid;
+            else hi = mid - 1;
+          }
+        const int l = lo;
+        const int e = (row - lstart[l]) * TS_RUNROW + (lane & 15);
+        const bool in = rin && e < lcnt[l];
         int32_t x = 0;
         uint32_t c = 0;
         if (in) {
-          x = D->col[0][base + e];
-          c = (uint32_t)D->cnt[base + e];
+          const int64_t pos = lbase[l] + e;
+          x = lcol0[l][pos];
+          c = (uint32_t)lcol1[l][pos];
         }
+        // Join variables four at a time: their value-table reads in flight together.
+        // Binding sums of 16 lanes fit 32 bits when every run count is below 2^28 / tr (a value's
+        // rows of one variable are at most tr).
+        const bool small = !__ballot(in && c >= (1u << 28) / (uint32_t)L->tr);
         for (int vg = 0; vg < nvars; vg += 4) {
-          const int vj = vg + (lane >> 4);
-          const int t = vj < nvars ? D->target[vj] : -1;
-          if (!__ballot(t >= 0)) continue;
-          uint32_t pn = 0;
-          if (in && t >= 0) {
-            const uint32_t h = ts_vhash(x, vj);
-            const uint32_t key = (((h >> 4) & 0x1ffffu) << 15) | ((uint32_t)vj << 11);
-            const int jc = jt[vj];
-            uint32_t slot = h >> (32 - TS_VLOG);
-            while (true) {
-              const uint32_t wd = vword[slot];
-              if (wd == 0) break;
-              if ((wd & TS_VKEY_MASK) == key && tstage[jc * TS_ROWS + (int)(wd & 0x7ffu) - 1] == x) {
-                pn = vpn[slot];
-                break;
+          int t[4];
+          unsigned long long key[4];
+          uint32_t slot[4], pn[4];
+          bool look[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int vj = vg + u;
+            t[u] = !rin || vj >= nvars ? -1 : vj < 4 ? ltgt[l][vj] : lists[l].target[vj];
+            key[u] = ts_vkey(x, vj);
+            slot[u] = ts_vhash(x, vj) >> (32 - TS_VLOG);
+            look[u] = in && t[u] >= 0;
+            pn[u] = 0;
+          }
+          while (look[0] || look[1] || look[2] || look[3]) {
+            unsigned long long wd[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) wd[u] = look[u] ? vtab[slot[u]] : 0ull;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              if (!look[u]) continue;
+              if (wd[u] == 0) {
+                look[u] = false;
+              } else if ((wd[u] & TS_VKEYBITS) == key[u]) {
+                // positive rows << 16 | negative rows
+                pn[u] = (uint32_t)((wd[u] >> TS_VPOS) & 0x1fffu) << 16 | (uint32_t)((wd[u] >> TS_VNEG) & 0x1fffu);
+                look[u] = false;
+              } else {
+                slot[u] = (slot[u] + 1) & (TS_VCAP - 1);
               }
-              slot = (slot + 1) & (TS_VCAP - 1);
             }
           }
-          // Row sums: covered (packed; a row's runs are distinct values, so its positive and
-          // negative rows stay below the chunk's rows) and binding in 21-bit pieces (a run count
-          // is below 2^31 and rows below 2^11, so each piece's sum over 16 lanes fits 32 bits).
-          const uint64_t bp = (uint64_t)c * (pn >> 16), bn = (uint64_t)c * (pn & 0xffffu);
-          const uint32_t cov = row_sum16(pn);
-          const uint32_t bp_lo = row_sum16((uint32_t)(bp & 0x1fffffu)), bp_hi = row_sum16((uint32_t)(bp >> 21));
-          const uint32_t bn_lo = row_sum16((uint32_t)(bn & 0x1fffffu)), bn_hi = row_sum16((uint32_t)(bn >> 21));
-          if ((lane & 15) == 15 && t >= 0 && cov != 0) {
-            atomicAdd(&acc[t][0], (unsigned long long)(cov >> 16));
-            atomicAdd(&acc[t][1], (unsigned long long)(cov & 0xffffu));
-            atomicAdd(&acc[t][2], ((unsigned long long)bp_hi << 21) + bp_lo);
-            atomicAdd(&acc[t][3], ((unsigned long long)bn_hi << 21) + bn_lo);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (!__ballot(t[u] >= 0 && pn[u] != 0)) continue;
+            // Row sums: covered (packed; a row's runs are distinct values of one list, so its
+            // positive and negative rows stay below the chunk's rows of the variable) and
+            // binding, in one 32-bit sum when every run count is small, else in 21-bit pieces (a
+            // run count is below 2^31 and rows below 2^11).
+            const uint32_t cov = row_sum16(pn[u]);
+            uint64_t sbp, sbn;
+            if (small) {
+              sbp = row_sum16(c * (pn[u] >> 16));
+              sbn = row_sum16(c * (pn[u] & 0xffffu));
+            } else {
+              const uint64_t bp = (uint64_t)c * (pn[u] >> 16), bn = (uint64_t)c * (pn[u] & 0xffffu);
+              sbp = ((uint64_t)row_sum16((uint32_t)(bp >> 21)) << 21) + row_sum16((uint32_t)(bp & 0x1fffffu));
+              sbn = ((uint64_t)row_sum16((uint32_t)(bn >> 21)) << 21) + row_sum16((uint32_t)(bn & 0x1fffffu));
+            }
+            if ((lane & 15) == 15 && t[u] >= 0 && cov != 0) {
+              atomicAdd(&acc[t[u]][0], (unsigned long long)(cov >> 16));
+              atomicAdd(&acc[t[u]][1], (unsigned long long)(cov & 0xffffu));
+              atomicAdd(&acc[t[u]][2], (unsigned long long)sbp);
+              atomicAdd(&acc[t[u]][3], (unsigned long long)sbn);
+            }
           }
         }
       } else {
-        // Tuples: TS_TUPCH / 64 per lane, their loads in flight together.
-        constexpr int J = TS_TUPCH / 64;
-        const int e0 = (k - lch[l]) * TS_TUPCH + lane;
-        const int32_t* c0p = D->col[0];
-        const int32_t* c1p = D->col[1];
-        int32_t x[J], y[J];
+        const int i0 = (k - rc) * TS_TUPCH + TS_TJ * lane;  // the lane's first tuple
+        if (i0 < nt)
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (lstart[mid] <= i0) lo = mid;
+            else hi = mid - 1;
+          }
+        int l = lo;
+        int32_t x[TS_TJ], y[TS_TJ];
+        int64_t pos[TS_TJ];
+        int lj[TS_TJ];
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-          const int e = e0 + 64 * j;
+        for (int j = 0; j < TS_TJ; ++j) {
+          const int i = i0 + j;
           x[j] = y[j] = 0;
-          if (e < n) {
-            x[j] = c0p[base + e];
-            y[j] = c1p[base + e];
+          pos[j] = -1;
+          lj[j] = l;
+          if (i < nt) {
+            while (lstart[l + 1] <= i) ++l;  // lists with no tuple in this cell are passed over
+            lj[j] = l;
+            pos[j] = lbase[l] + (i - lstart[l]);
+            x[j] = lcol0[l][pos[j]];
+            y[j] = lcol1[l][pos[j]];
           }
         }
+        ts_pair_probes(a, lists, lj, pos, x, y, pword, tstage, rflag, jt, copy);
 #pragma unroll
-        for (int j = 0; j < J; ++j)
-          if (e0 + 64 * j < n) {
-            ts_pair_probe(a, D, base + e0 + 64 * j, x[j], y[j], pword, tstage, rflag, jt, copy);
-            if (x[j] == y[j]) ts_equal_pair(a, D, base + e0 + 64 * j, x[j], nvars, seg_off, tstage, rflag, jt, copy);
-          }
+        for (int j = 0; j < TS_TJ; ++j)
+          if (pos[j] >= 0 && x[j] == y[j])
+            ts_equal_pair(a, lists + lj[j], pos[j], x[j], nvars, seg_off, tstage, rflag, jt, copy);
       }
     }
   }
@@ -831,10 +999,15 @@ __global__ void k_tuple_compact(TsCols s, int64_t n, const int64_t* flags, const
   }
 }
 
-// Tuple multiplicities from consecutive first rows; the last tuple ends at row n.
-__global__ void k_tuple_counts(const int32_t* first_row, int64_t tuples, int64_t n, int32_t* cnt) {
+// Tuple multiplicities from consecutive first rows (the last tuple ends at row n); with three or
+// more key columns, bit 31 marks a tuple whose first two columns differ from the previous
+// tuple's (the first tuple of a (column 0, column 1) group).
+__global__ void k_tuple_counts(TsOut tup, int m, const int32_t* first_row, int64_t tuples, int64_t n, int32_t* cnt) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < tuples) cnt[t] = (int32_t)((t + 1 < tuples ? (int64_t)first_row[t + 1] : n) - first_row[t]);
+  if (t >= tuples) return;
+  uint32_t c = (uint32_t)((t + 1 < tuples ? (int64_t)first_row[t + 1] : n) - first_row[t]);
+  if (m >= 3 && (t == 0 || tup.c[0][t] != tup.c[0][t - 1] || tup.c[1][t] != tup.c[1][t - 1])) c |= 0x80000000u;
+  cnt[t] = (int32_t)c;
 }
 
 // One thread per work item (the grid covers an upper bound of the item count): item i belongs to
@@ -1683,6 +1856,8 @@ static std::shared_ptr<TupleList> relation_tuples(Context* ctx, RelPart& rp, con
   const int64_t n = rp.pt->num_rows;
   const int64_t parts = (int64_t)1 << rp.pt->radix_bits;
   auto tl = std::make_shared<TupleList>();
+  QF_REQUIRE(n < ((int64_t)1 << 31), "relation rows must stay below 2^31");
+
   std::vector<DevBuf> buf[2];
   for (int s = 0; s < 2; ++s)
     for (int k = 0; k < m; ++k) buf[s].emplace_back(ctx, std::max<int64_t>(1, n) * 4);
@@ -1727,7 +1902,7 @@ static std::shared_ptr<TupleList> relation_tuples(Context* ctx, RelPart& rp, con
                      rp.pt->offsets.as<int64_t>(), parts, tl->off.as<int64_t>());
   QF_CHECK_HIP(hipGetLastError());
   if (tuples > 0) {
-    hipLaunchKernelGGL(k_tuple_counts, dim3((unsigned)ceil_div(tuples, 256)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_tuple_counts, dim3((unsigned)ceil_div(tuples, 256)), dim3(256), 0, ctx->stream, out, m,
                        first.as<int32_t>(), tuples, n, tl->cnt.as<int32_t>());
     QF_CHECK_HIP(hipGetLastError());
   }
@@ -1832,7 +2007,7 @@ struct StepPlan {
     int64_t items_bound = 0;
   };
   std::vector<StreamLaunch> slaunches;      // units of the streamed form, in launches of bounded lists
-  DevBuf d_ts_cells, d_ts_counts, d_ts_starts, d_ts_items, d_ts_num, d_light;
+  DevBuf d_ts_counts, d_ts_starts, d_ts_parts, d_cellmat, d_ts_items, d_ts_num, d_light;
   int64_t ts_items = 0;                     // upper bound of any streamed launch's work items
   size_t ts_lds = 0;
   int ts_blocks = 0;
@@ -2080,14 +2255,31 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
         run.cnt = tl->cnt.as<int32_t>();
         run.off = tl->off.as<int64_t>();
         run.unit = unit_index;
+        run.psubset = -1;
         ust.lists.push_back(run);
         ust.sizes.push_back(tl->tuples);
       }
+      // List keys: the join column, then the subset's other columns ascending.  A two-column
+      // subset whose columns begin a wider subset's key is counted from that subset's list.
+      std::vector<std::vector<int>> keys(subsets.size());
       for (int s = 1; s < (int)subsets.size(); ++s) {
+        keys[s].push_back(jc);
+        for (int c : subsets[s].first)
+          if (c != jc) keys[s].push_back(c);
+      }
+      std::vector<int> merged_into(subsets.size(), -1), prefix_of(subsets.size(), -1);
+      for (int s = 1; s < (int)subsets.size(); ++s) {
+        if (keys[s].size() != 2) continue;
+        for (int t = 1; t < (int)subsets.size() && merged_into[s] < 0; ++t)
+          if (keys[t].size() >= 3 && keys[t][1] == keys[s][1] && prefix_of[t] < 0) {
+            merged_into[s] = t;
+            prefix_of[t] = s;
+          }
+      }
+      for (int s = 1; s < (int)subsets.size(); ++s) {
+        if (merged_into[s] >= 0) continue;
         const std::vector<int>& pc = subsets[s].first;  // ascending relation columns, the join column among them
-        std::vector<int> key{jc};
-        for (int c : pc)
-          if (c != jc) key.push_back(c);
+        const std::vector<int>& key = keys[s];
         auto tl = relation_tuples(ctx, rp, key);
         P.keep_tuples.push_back(tl);
         TsList L{};
@@ -2100,11 +2292,19 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
         L.off = tl->off.as<int64_t>();
         L.unit = unit_index;
         L.subset = s;
+        L.psubset = prefix_of[s];
+        if (L.psubset >= 0) {
+          const std::vector<int>& ppc = subsets[L.psubset].first;
+          L.pkidx1 = (int)(std::find(ppc.begin(), ppc.end(), key[1]) - ppc.begin());
+        }
         for (int v = 0; v < EV_MAXG; ++v) L.target[v] = L.vgroup[v] = -1;
         for (int gi = 0; gi < du.ngroups; ++gi) {
           const int vi = var_index(S.groups[gi]->join_var);
           L.vgroup[vi] = gi;
           for (int li = du.g[gi].sbeg[s]; li < du.g[gi].sbeg[s + 1]; ++li) ust.wmask[vi] |= 1u << P.lits[li].tcol[L.kidx[1]];
+          if (L.psubset >= 0)
+            for (int li = du.g[gi].sbeg[L.psubset]; li < du.g[gi].sbeg[L.psubset + 1]; ++li)
+              ust.wmask[vi] |= 1u << P.lits[li].tcol[L.pkidx1];
         }
         ust.lists.push_back(L);
         ust.sizes.push_back(tl->tuples);
@@ -2210,22 +2410,32 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     struct Pack {
       std::vector<TsList> lists;
       std::vector<int64_t> sizes;
+      int nrun = 0;
+      int64_t entries = 0;
       std::vector<TsTarget> targets;
       uint32_t wmask[EV_MAXG] = {};
     };
     std::vector<Pack> packs(1);
     for (const UnitStream& u : ustreams) {
+      int64_t entries = 0;
+      for (int64_t s : u.sizes) entries += s;
+      QF_REQUIRE(entries < ((int64_t)1 << 31), "a unit's key tuples must stay below 2^31");
       if (packs.back().lists.size() + u.lists.size() > (size_t)TS_MAXL ||
-          packs.back().targets.size() + u.targets.size() > (size_t)TS_MAXT)
+          packs.back().targets.size() + u.targets.size() > (size_t)TS_MAXT ||
+          packs.back().entries + entries >= ((int64_t)1 << 31))
         packs.emplace_back();
       Pack& pk = packs.back();
       const int t0 = (int)pk.targets.size();
-      for (TsList L : u.lists) {
+      for (size_t i = 0; i < u.lists.size(); ++i) {
+        TsList L = u.lists[i];
         for (int v = 0; v < EV_MAXG; ++v)
           if (L.target[v] >= 0) L.target[v] += t0;
-        pk.lists.push_back(L);
+        // Run lists first: the kernel concatenates a cell's runs, then its tuples.
+        const size_t at = L.m == 1 ? (size_t)pk.nrun++ : pk.lists.size();
+        pk.lists.insert(pk.lists.begin() + at, L);
+        pk.sizes.insert(pk.sizes.begin() + at, u.sizes[i]);
+        pk.entries += u.sizes[i];
       }
-      pk.sizes.insert(pk.sizes.end(), u.sizes.begin(), u.sizes.end());
       pk.targets.insert(pk.targets.end(), u.targets.begin(), u.targets.end());
       for (int v = 0; v < EV_MAXG; ++v) pk.wmask[v] |= u.wmask[v];
     }
@@ -2247,6 +2457,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       TsLaunch tl{};
       tl.lists = SL.d_lists.as<TsList>();
       tl.nlists = (int)pk.lists.size();
+      tl.nrun = pk.nrun;
       tl.targets = SL.d_targets.as<TsTarget>();
       tl.ntargets = (int)pk.targets.size();
       tl.nvars = nvars;
@@ -2262,17 +2473,18 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       SL.d_launch = DevBuf(ctx, sizeof(TsLaunch));
       copy_h2d(ctx, SL.d_launch.get(), &tl, sizeof(TsLaunch));
       // Items: at most TS_CMAX binding chunks per cell times its parts, which number at most
-      // one per cell with entries plus the cell's wave chunks over TS_PART.
-      int64_t chunks = 0;
-      for (size_t i = 0; i < pk.lists.size(); ++i) chunks += ceil_div(pk.sizes[i], ts_chunk(pk.lists[i].m)) + cells;
+      // one per cell with entries plus the cell's wave chunks over TS_PART (a cell's chunks are
+      // at most its entries / TS_RUNROW, plus one row per run list, over TS_RUNROWS, plus two).
+      const int64_t chunks = (pk.entries / TS_RUNROW + cells * (int64_t)pk.nrun) / TS_RUNROWS + 2 * cells;
       SL.items_bound = (int64_t)TS_CMAX * (cells + chunks / TS_PART + 1);
       bound = std::max(bound, SL.items_bound);
       P.slaunches.push_back(std::move(SL));
     }
-    const int64_t cell_range[2] = {0, cells};
-    P.d_ts_cells = DevBuf(ctx, sizeof(cell_range));
-    copy_h2d(ctx, P.d_ts_cells.get(), cell_range, sizeof(cell_range));
+    size_t max_lists = 1;
+    for (const Pack& pk : packs) max_lists = std::max(max_lists, pk.lists.size());
     P.d_ts_counts = DevBuf(ctx, cells * 8);
+    P.d_ts_parts = DevBuf(ctx, cells * 4);
+    P.d_cellmat = DevBuf(ctx, (size_t)cells * max_lists * sizeof(int2));
     P.d_ts_starts = DevBuf(ctx, cells * 8);
     P.d_light = DevBuf(ctx, cells);
     P.ts_items = bound;
@@ -2378,17 +2590,18 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
       {
         PhaseScope ps(ctx, "work_items");
         hipLaunchKernelGGL(k_ts_count, dim3((unsigned)ceil_div(cells_T, 256)), dim3(256), 0, ctx->stream,
-                           SL.d_launch.as<TsLaunch>(), P.d_ts_counts.as<int64_t>(),
-                           li == 0 ? P.d_light.as<uint8_t>() : nullptr);
+                           SL.d_launch.as<TsLaunch>(), P.d_ts_counts.as<int64_t>(), P.d_ts_parts.as<int32_t>(),
+                           P.d_cellmat.as<int2>(), li == 0 ? P.d_light.as<uint8_t>() : nullptr);
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), cells_T);
-        hipLaunchKernelGGL(k_expand_items, dim3((unsigned)ceil_div(SL.items_bound, 256)), dim3(256), 0, ctx->stream,
-                           P.d_ts_cells.as<int64_t>(), 1, P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(),
+        hipLaunchKernelGGL(k_ts_expand, dim3((unsigned)ceil_div(SL.items_bound, 256)), dim3(256), 0, ctx->stream,
+                           P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), P.d_ts_parts.as<int32_t>(), cells_T,
                            P.d_ts_items.as<DevItem>(), P.d_ts_num.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
       }
       TsArgs ta{};
       ta.launch = SL.d_launch.as<TsLaunch>();
+      ta.cellmat = P.d_cellmat.as<int2>();
       ta.units = P.d_units.as<DevUnit>();
       ta.lits = P.d_lits.as<DevLit>();
       ta.items = P.d_ts_items.as<DevItem>();
