@@ -273,6 +273,12 @@ def main():
         pr, tr, best_ms = choice["partition_rows"], choice["tile_rows"], choice["best_ms"]
         tune.update(partition_rows=pr, tile_rows=tr)
         log("autotuner: %d facts per partition, %d-row tiles (%.2f ms)" % (pr, tr, best_ms))
+    # Eager calls (no graph) with the configuration the graph replays use: the autotuned one
+    # (looked up in the context's autotuner cache, which also holds whether the streamed form won)
+    # or the one given.
+    auto_cfg = not args.no_autotune and not args.radix_bits and not args.tile_rows
+    eager_kw = (dict(autotune=True) if auto_cfg else
+                dict(radix_bits=tune["radix_bits"], tile_rows=tune["tile_rows"], partition_rows=tune["partition_rows"]))
     for _ in range(max(0, args.warmup - 1)):
         step("batched")
     torch.cuda.synchronize()
@@ -292,8 +298,7 @@ def main():
     ctx.reset_timing()
     for i in range(args.steps):
         eager_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], lits,
-                                                mode="batched", reuse_partitions=True, radix_bits=tune["radix_bits"],
-                                                tile_rows=tune["tile_rows"], partition_rows=tune["partition_rows"])
+                                                mode="batched", reuse_partitions=True, **eager_kw)
     torch.cuda.synchronize()
     assert np.array_equal(eager_counts, ref_counts), "eager and graph counts differ"
     phases = ctx.phase_times()
@@ -308,8 +313,7 @@ def main():
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             fresh_counts, _ = ctx.evaluate_literals(fresh, w["num_positive"], w["relations"], lits, mode="batched",
-                                                    reuse_partitions=True, radix_bits=tune["radix_bits"],
-                                                    tile_rows=tune["tile_rows"], partition_rows=tune["partition_rows"])
+                                                    reuse_partitions=True, **eager_kw)
             times.append(time.perf_counter() - t0)
             assert np.array_equal(fresh_counts, ref_counts), "counts on a new binding table differ"
             del fresh

@@ -361,7 +361,10 @@ __global__ void k_count_items(const DevBatch* batches, const DevUnit* units, con
 //    with r in the rows of join variable v is a match of the literals of (u, subset, group of v)
 //    whose column k1 is bound to w and whose further columns equal row r: binding += c and
 //    covered += 1 (distinct tuples, so each (literal, row) pair matches at most one tuple).
-// Cells whose binding rows exceed TS_CMAX chunks are left to k_eval.
+// A cell is streamed when its binding rows fit TS_CMAX chunks and its relation entries (streamed
+// once per chunk) stay within TS_RATIO per binding row; the others (a heavily repeated binding
+// value, or a few binding rows against a hot relation value) are left to k_eval, whose cost
+// follows the binding rows.
 
 constexpr int TS_THREADS = 512;
 constexpr int TS_ROWS = 1024;          // staged binding rows per work item (all join variables)
@@ -374,6 +377,7 @@ constexpr int TS_MAXL = 128;           // entry lists per launch
 constexpr int TS_MAXT = 256;           // (unit, join variable) counters of join-column literals per launch
 constexpr int TS_PART = 512;           // wave chunks of relation entries per work item
 constexpr int TS_CMAX = 4;             // binding chunks of a cell that the streamed form takes
+constexpr int TS_RATIO = 32;           // relation entries per binding row that it takes
 constexpr int TS_RUNROW = 16;          // runs per row of lanes (a row's runs belong to one list)
 constexpr int TS_RUNROWS = 4;          // rows per wave chunk
 constexpr int TS_TUPCH = 256;          // tuples per wave chunk (four per lane)
@@ -420,12 +424,32 @@ struct TsLaunch {
 };
 
 
-// Work items per cell: (binding chunks) x (parts of the cell's wave chunks: its runs, all run
-// lists one after another in rows of TS_RUNROW, TS_RUNROWS rows per chunk, then its tuples in
-// chunks of TS_TUPCH); light[q] = 1 when the cell's binding rows fit TS_CMAX chunks (k_eval then
-// skips it).  cellmat[q * nlists + l] = (first entry, entries) of list l in cell q, so that a work
-// item reads its cell's lists in one pass.
-__global__ void k_ts_count(const TsLaunch* L_, int64_t* counts, int32_t* parts, int2* cellmat, uint8_t* light) {
+// Which cells the streamed form takes (light[q] = 1; k_eval skips them), over the entry lists of
+// every streamed launch (all launches stage the same binding rows).
+__global__ void k_ts_light(const TsLaunch* const* launches, int nlaunches, uint8_t* light) {
+  cptr<TsLaunch> L0 = as_const(launches[0]);
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L0->cells) return;
+  int64_t chunks = 0, rows = 0;
+  for (int v = 0; v < L0->nvars; ++v) {
+    const int64_t n = L0->toff[v][q + 1] - L0->toff[v][q];
+    chunks = max(chunks, (n + L0->tr - 1) / L0->tr);
+    rows += n;
+  }
+  int64_t entries = 0;
+  for (int i = 0; i < nlaunches; ++i) {
+    cptr<TsLaunch> L = as_const(launches[i]);
+    cptr<TsList> lists = as_const(L->lists);
+    for (int l = 0; l < L->nlists; ++l) entries += lists[l].off[q + 1] - lists[l].off[q];
+  }
+  light[q] = (chunks <= TS_CMAX && chunks * entries <= (int64_t)TS_RATIO * rows) ? 1 : 0;
+}
+
+// Work items per cell that the streamed form takes: (binding chunks) x (parts of the cell's wave
+// chunks: its runs, all run lists one after another in rows of TS_RUNROW, TS_RUNROWS rows per
+// chunk, then its tuples in chunks of TS_TUPCH).  cellmat[q * nlists + l] = (first entry,
+// entries) of list l in cell q, so that a work item reads its cell's lists in one pass.
+__global__ void k_ts_count(const TsLaunch* L_, const uint8_t* light, int64_t* counts, int32_t* parts, int2* cellmat) {
   cptr<TsLaunch> L = as_const(L_);
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= L->cells) return;
@@ -434,8 +458,6 @@ __global__ void k_ts_count(const TsLaunch* L_, int64_t* counts, int32_t* parts, 
     const int64_t rows = L->toff[v][q + 1] - L->toff[v][q];
     chunks = max(chunks, (rows + L->tr - 1) / L->tr);
   }
-  const bool lt = chunks <= TS_CMAX;
-  if (light) light[q] = lt ? 1 : 0;
   cptr<TsList> lists = as_const(L->lists);
   int64_t rows = 0, nt = 0;
   for (int l = 0; l < L->nlists; ++l) {
@@ -447,7 +469,7 @@ __global__ void k_ts_count(const TsLaunch* L_, int64_t* counts, int32_t* parts, 
   const int64_t wc = (rows + TS_RUNROWS - 1) / TS_RUNROWS + (nt + TS_TUPCH - 1) / TS_TUPCH;
   const int64_t p = (wc + TS_PART - 1) / TS_PART;
   parts[q] = (int32_t)max<int64_t>(1, p);
-  counts[q] = (lt && chunks > 0) ? chunks * p : 0;
+  counts[q] = light[q] ? chunks * p : 0;
 }
 
 // One thread per work item: item i of cell q (the last cell whose scanned start is <= i) is
@@ -1761,10 +1783,15 @@ static int auto_radix_bits(int64_t rows, int target) {
   return r;
 }
 
-// Autotuner state: best (partition_rows, tile_rows) per workload shape.
+// Autotuner state: best (partition_rows, tile_rows, streamed form on) per workload shape.
+struct TuneChoice {
+  int partition_rows = 0, tile_rows = 0;
+  bool stream = true;
+  double ms = 0.0;
+};
 struct TuneCache {
-  std::map<std::string, std::tuple<int, int, double>> best;
-  std::tuple<int, int, double> last{0, 0, 0.0};
+  std::map<std::string, TuneChoice> best;
+  TuneChoice last;
 };
 
 // HBM count tables of the heavy partitions of one (relation, join column) for one subset.
@@ -2008,6 +2035,7 @@ struct StepPlan {
   };
   std::vector<StreamLaunch> slaunches;      // units of the streamed form, in launches of bounded lists
   DevBuf d_ts_counts, d_ts_starts, d_ts_parts, d_cellmat, d_ts_items, d_ts_num, d_light;
+  DevBuf d_ts_launches;  // the launches' TsLaunch addresses
   int64_t ts_items = 0;                     // upper bound of any streamed launch's work items
   size_t ts_lds = 0;
   int ts_blocks = 0;
@@ -2487,6 +2515,10 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     P.d_cellmat = DevBuf(ctx, (size_t)cells * max_lists * sizeof(int2));
     P.d_ts_starts = DevBuf(ctx, cells * 8);
     P.d_light = DevBuf(ctx, cells);
+    std::vector<const TsLaunch*> la;
+    for (const auto& SL : P.slaunches) la.push_back(SL.d_launch.as<TsLaunch>());
+    P.d_ts_launches = DevBuf(ctx, la.size() * sizeof(const TsLaunch*));
+    copy_h2d(ctx, P.d_ts_launches.get(), la.data(), la.size() * sizeof(const TsLaunch*));
     P.ts_items = bound;
     P.d_ts_items = DevBuf(ctx, bound * sizeof(DevItem));
     P.d_ts_num = DevBuf(ctx, 8);
@@ -2583,15 +2615,21 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
         }
       }
     }
-    // Streamed units, launch by launch; the first launch marks the light cells.
+    // Streamed units: the cells they take, then launch by launch.
     const int64_t cells_T = (int64_t)1 << P.R_T;
+    if (!P.slaunches.empty()) {
+      PhaseScope ps(ctx, "work_items");
+      hipLaunchKernelGGL(k_ts_light, dim3((unsigned)ceil_div(cells_T, 256)), dim3(256), 0, ctx->stream,
+                         P.d_ts_launches.as<const TsLaunch*>(), (int)P.slaunches.size(), P.d_light.as<uint8_t>());
+      QF_CHECK_HIP(hipGetLastError());
+    }
     for (size_t li = 0; li < P.slaunches.size(); ++li) {
       const StepPlan::StreamLaunch& SL = P.slaunches[li];
       {
         PhaseScope ps(ctx, "work_items");
         hipLaunchKernelGGL(k_ts_count, dim3((unsigned)ceil_div(cells_T, 256)), dim3(256), 0, ctx->stream,
-                           SL.d_launch.as<TsLaunch>(), P.d_ts_counts.as<int64_t>(), P.d_ts_parts.as<int32_t>(),
-                           P.d_cellmat.as<int2>(), li == 0 ? P.d_light.as<uint8_t>() : nullptr);
+                           SL.d_launch.as<TsLaunch>(), P.d_light.as<uint8_t>(), P.d_ts_counts.as<int64_t>(),
+                           P.d_ts_parts.as<int32_t>(), P.d_cellmat.as<int2>());
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), cells_T);
         hipLaunchKernelGGL(k_ts_expand, dim3((unsigned)ceil_div(SL.items_bound, 256)), dim3(256), 0, ctx->stream,
@@ -2681,7 +2719,7 @@ struct GraphCache {
 
 static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
                             std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o,
-                            uint64_t* h_counts, uint64_t* d_bits);
+                            uint64_t* h_counts, uint64_t* d_bits, bool allow_stream = true);
 
 static TuneCache& tune_cache(Context* ctx) {
   if (!ctx->tune_cache) ctx->tune_cache = std::make_shared<TuneCache>();
@@ -2690,16 +2728,17 @@ static TuneCache& tune_cache(Context* ctx) {
 
 // Times one configuration of the step (device work only, HIP events on the context stream).
 static double time_config(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
-                          std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, uint64_t* h_counts) {
+                          std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, uint64_t* h_counts,
+                          bool stream) {
   qf_eval_options t = o;
   t.autotune = 0;
   t.use_graph = 0;
-  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, nullptr);  // warm caches
+  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, nullptr, stream);  // warm caches
   hipEvent_t e0, e1;
   QF_CHECK_HIP(hipEventCreate(&e0));
   QF_CHECK_HIP(hipEventCreate(&e1));
   QF_CHECK_HIP(hipEventRecord(e0, ctx->stream));
-  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, nullptr);
+  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, nullptr, stream);
   QF_CHECK_HIP(hipEventRecord(e1, ctx->stream));
   QF_CHECK_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -2726,34 +2765,33 @@ static void run_groups(Context* ctx, const qf_table& binding, int64_t num_positi
   TuneCache& tc = tune_cache(ctx);
   auto it = tc.best.find(shape);
   if (it == tc.best.end()) {
-    // Grid: facts per relation partition (radix bits) x binding rows per work item.
-    double best_ms = 1e30;
-    int best_pr = 256, best_tile = 1024;
-    for (int pr : {128, 256, 512})
-      for (int tile : {512, 768, 1024, 1536, 2048}) {
-        qf_eval_options t = o;
-        t.partition_rows = pr;
-        t.tile_rows = tile;
-        const double ms = time_config(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts);
-        if (ms < best_ms) {
-          best_ms = ms;
-          best_pr = pr;
-          best_tile = tile;
+    // Grid: facts per relation partition (radix bits) x binding rows per work item, with the
+    // streamed form on or off (the batched form without covered bitvectors uses it).
+    TuneChoice best;
+    best.ms = 1e30;
+    const bool streamable = o.mode == QF_EVAL_BATCHED && d_bits == nullptr;
+    for (int s = streamable ? 1 : 0; s >= 0; --s)
+      for (int pr : {128, 256, 512})
+        for (int tile : {512, 768, 1024, 1536, 2048}) {
+          qf_eval_options t = o;
+          t.partition_rows = pr;
+          t.tile_rows = tile;
+          const double ms = time_config(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, s != 0);
+          if (ms < best.ms) best = TuneChoice{pr, tile, s != 0, ms};
         }
-      }
-    it = tc.best.emplace(shape, std::make_tuple(best_pr, best_tile, best_ms)).first;
+    it = tc.best.emplace(shape, best).first;
   }
   tc.last = it->second;
   qf_eval_options t = o;
   t.autotune = 0;
-  t.partition_rows = std::get<0>(it->second);
-  t.tile_rows = std::get<1>(it->second);
-  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, d_bits);
+  t.partition_rows = it->second.partition_rows;
+  t.tile_rows = it->second.tile_rows;
+  run_groups_impl(ctx, binding, num_positive, relations, specs, num_slots, t, h_counts, d_bits, it->second.stream);
 }
 
 static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
                             std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o,
-                            uint64_t* h_counts, uint64_t* d_bits) {
+                            uint64_t* h_counts, uint64_t* d_bits, bool allow_stream) {
   if (num_slots == 0) return;
   // A captured step is replayed when the same call repeats: same binding and relation
   // columns, literal set, options and output pointer.
@@ -2764,7 +2802,7 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
   if (o.use_graph && o.mode == QF_EVAL_BATCHED) {
     key.reserve(64 + 16 * (size_t)num_slots);
     key.insert(key.end(), {binding.num_rows, num_positive, o.radix_bits, o.tile_rows, o.partition_rows,
-                           o.reuse_partitions, (int64_t)(uintptr_t)d_bits, binding.num_columns});
+                           o.reuse_partitions, (int64_t)(uintptr_t)d_bits, binding.num_columns, allow_stream ? 1 : 0});
     for (int c = 0; c < binding.num_columns; ++c) key.push_back((int64_t)(uintptr_t)binding.columns[c]);
     key.push_back((int64_t)specs.size());
     for (const GroupSpec& g : specs) {
@@ -2794,7 +2832,7 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
     }
   }
   std::unique_ptr<StepPlan> P(new StepPlan);
-  plan_step(ctx, binding, relations, specs, num_slots, o, *P, o.mode == QF_EVAL_BATCHED && d_bits == nullptr);
+  plan_step(ctx, binding, relations, specs, num_slots, o, *P, allow_stream && o.mode == QF_EVAL_BATCHED && d_bits == nullptr);
   QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->pinned), (size_t)num_slots * 4 * sizeof(uint64_t),
                              hipHostMallocDefault));
   if (!key.empty()) {
@@ -2864,9 +2902,9 @@ void qf_eval_options_default(qf_eval_options* o) {
 int qf_autotune_choice(qf_context* c, int* partition_rows, int* tile_rows, double* best_ms) {
   return guarded(c, [&](Context* ctx) {
     const TuneCache& tc = tune_cache(ctx);
-    if (partition_rows) *partition_rows = std::get<0>(tc.last);
-    if (tile_rows) *tile_rows = std::get<1>(tc.last);
-    if (best_ms) *best_ms = std::get<2>(tc.last);
+    if (partition_rows) *partition_rows = tc.last.partition_rows;
+    if (tile_rows) *tile_rows = tc.last.tile_rows;
+    if (best_ms) *best_ms = tc.last.ms;
   });
 }
 
