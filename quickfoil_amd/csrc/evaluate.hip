@@ -2773,6 +2773,7 @@ static void run_groups(Context* ctx, const qf_table& binding, int64_t num_positi
     for (int s = streamable ? 1 : 0; s >= 0; --s)
       for (int pr : {128, 256, 512})
         for (int tile : {512, 768, 1024, 1536, 2048}) {
+          if (s && tile != 768 && tile != 1536) continue;  // with the streamed form the tile sizes only k_eval's rest
           qf_eval_options t = o;
           t.partition_rows = pr;
           t.tile_rows = tile;
