@@ -478,7 +478,10 @@ __global__ void k_ts_expand(const int64_t* counts, const int64_t* starts, const 
                             DevItem* items, int64_t* total) {
   const int64_t all = starts[cells - 1] + counts[cells - 1];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *total = all;
+  if (i == 0) {
+    total[0] = all;
+    total[1] = 0;  // the next item to hand out (k_stream's workgroups take items as they finish)
+  }
   if (i >= all) return;
   int64_t lo = 0, hi = cells - 1;
   while (lo < hi) {
@@ -514,8 +517,8 @@ struct TsArgs {
   const DevUnit* units;
   const DevLit* lits;
   const DevItem* items;
-  const int64_t* num_items;
-  int64_t max_items;  // the item list's capacity
+  int64_t* num_items;  // [0]: items; [1]: the next item to hand out
+  int64_t max_items;   // the item list's capacity
   int num_positive;
   unsigned long long* counts_out;  // ncopies x num_slots x 4: block b adds into copy b % ncopies
   int64_t copy_words;
@@ -665,10 +668,15 @@ __global__ __launch_bounds__(TS_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   const int lane = threadIdx.x & 63;
   const int wave = uni(threadIdx.x >> 6);
   const int nvars = L->nvars, nlists = L->nlists, nrun = L->nrun;
-  const int64_t total = min(uni64(*a.num_items), a.max_items);
+  const int64_t total = min(uni64(a.num_items[0]), a.max_items);
   cptr<DevItem> items = as_const(a.items);
   cptr<TsList> lists = as_const(L->lists);
   unsigned long long* copy = a.counts_out + (int64_t)(blockIdx.x % a.ncopies) * a.copy_words;
+  // Items are handed out one at a time from a device counter (their costs vary with the cells);
+  // thread 0 asks for the next one while the current one streams.
+  __shared__ int64_t item_next;
+  int64_t next = 0;
+  if (threadIdx.x == 0) next = (int64_t)atomicAdd(reinterpret_cast<unsigned long long*>(a.num_items + 1), 1ull);
   for (int i = threadIdx.x; i < TS_MAXT * 4; i += TS_THREADS) (&acc[0][0])[i] = 0;
   for (int i = threadIdx.x; i < nlists; i += TS_THREADS) {
     lcol0[i] = lists[i].col[0];
@@ -682,10 +690,13 @@ __global__ __launch_bounds__(TS_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
     toff[threadIdx.x] = L->toff[threadIdx.x];
   }
 
-  for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+  while (true) {
+    if (threadIdx.x == 0) item_next = next;
+    __syncthreads();  // the previous item is done with the tables, rows and lists
+    const int64_t it = item_next;
+    if (it >= total) break;
     const int64_t q = items[it].q;
     const int chunk = items[it].chunk, part = items[it].batch;  // binding chunk, part of the cell's entries
-    __syncthreads();  // the previous item is done with the tables, rows and lists
     // Binding rows of the chunk: the join variables' segments one after another (wave-uniform
     // loop); each thread notes the segment and source row of its two staged rows.
     constexpr int RPT = TS_ROWS / TS_THREADS;  // staged rows per thread
@@ -851,6 +862,7 @@ __global__ __launch_bounds__(TS_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
       }
     }
     __syncthreads();
+    if (threadIdx.x == 0) next = (int64_t)atomicAdd(reinterpret_cast<unsigned long long*>(a.num_items + 1), 1ull);
     // Stream the item's wave chunks, each wave taking the next one as it finishes: run chunks of
     // TS_RUNROWS rows of TS_RUNROW lanes (a row holds runs of one list), then tuple chunks of
     // TS_TUPCH tuples (TS_TJ consecutive ones per lane).  A lane finds its list by binary search
@@ -2521,7 +2533,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     copy_h2d(ctx, P.d_ts_launches.get(), la.data(), la.size() * sizeof(const TsLaunch*));
     P.ts_items = bound;
     P.d_ts_items = DevBuf(ctx, bound * sizeof(DevItem));
-    P.d_ts_num = DevBuf(ctx, 8);
+    P.d_ts_num = DevBuf(ctx, 16);
     P.ts_lds = (size_t)P.bcols.size() * TS_ROWS * 4 + (size_t)(2 * TS_VCAP + TS_PCAP) * 4 + TS_ROWS;
     QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_stream),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.ts_lds));
