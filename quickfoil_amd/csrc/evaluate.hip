@@ -361,10 +361,11 @@ __global__ void k_count_items(const DevBatch* batches, const DevUnit* units, con
 //    with r in the rows of join variable v is a match of the literals of (u, subset, group of v)
 //    whose column k1 is bound to w and whose further columns equal row r: binding += c and
 //    covered += 1 (distinct tuples, so each (literal, row) pair matches at most one tuple).
-// A cell is streamed when its binding rows fit TS_CMAX chunks and its relation entries (streamed
-// once per chunk) stay within TS_RATIO per binding row; the others (a heavily repeated binding
-// value, or a few binding rows against a hot relation value) are left to k_eval, whose cost
-// follows the binding rows.
+// A cell is streamed when its binding rows fit TS_CMAX chunks (its relation entries are streamed
+// once per chunk); the others (heavily repeated binding values) are left to k_eval, whose cost
+// follows the binding rows.  Whether a workload is better off without the streamed form (many
+// binding rows per cell against few relation entries, or hot values on both sides) is the
+// autotuner's choice.
 
 constexpr int TS_THREADS = 512;
 constexpr int TS_ROWS = 1024;          // staged binding rows per work item (all join variables)
@@ -377,7 +378,6 @@ constexpr int TS_MAXL = 128;           // entry lists per launch
 constexpr int TS_MAXT = 256;           // (unit, join variable) counters of join-column literals per launch
 constexpr int TS_PART = 512;           // wave chunks of relation entries per work item
 constexpr int TS_CMAX = 4;             // binding chunks of a cell that the streamed form takes
-constexpr int TS_RATIO = 32;           // relation entries per binding row that it takes
 constexpr int TS_RUNROW = 16;          // runs per row of lanes (a row's runs belong to one list)
 constexpr int TS_RUNROWS = 4;          // rows per wave chunk
 constexpr int TS_TUPCH = 256;          // tuples per wave chunk (four per lane)
@@ -424,25 +424,15 @@ struct TsLaunch {
 };
 
 
-// Which cells the streamed form takes (light[q] = 1; k_eval skips them), over the entry lists of
-// every streamed launch (all launches stage the same binding rows).
-__global__ void k_ts_light(const TsLaunch* const* launches, int nlaunches, uint8_t* light) {
-  cptr<TsLaunch> L0 = as_const(launches[0]);
+// Which cells the streamed form takes (light[q] = 1; k_eval skips them): those whose binding rows
+// fit TS_CMAX chunks (every streamed launch stages the same binding rows).
+__global__ void k_ts_light(const TsLaunch* L_, uint8_t* light) {
+  cptr<TsLaunch> L = as_const(L_);
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= L0->cells) return;
-  int64_t chunks = 0, rows = 0;
-  for (int v = 0; v < L0->nvars; ++v) {
-    const int64_t n = L0->toff[v][q + 1] - L0->toff[v][q];
-    chunks = max(chunks, (n + L0->tr - 1) / L0->tr);
-    rows += n;
-  }
-  int64_t entries = 0;
-  for (int i = 0; i < nlaunches; ++i) {
-    cptr<TsLaunch> L = as_const(launches[i]);
-    cptr<TsList> lists = as_const(L->lists);
-    for (int l = 0; l < L->nlists; ++l) entries += lists[l].off[q + 1] - lists[l].off[q];
-  }
-  light[q] = (chunks <= TS_CMAX && chunks * entries <= (int64_t)TS_RATIO * rows) ? 1 : 0;
+  if (q >= L->cells) return;
+  int64_t chunks = 0;
+  for (int v = 0; v < L->nvars; ++v) chunks = max(chunks, (L->toff[v][q + 1] - L->toff[v][q] + L->tr - 1) / L->tr);
+  light[q] = chunks <= TS_CMAX ? 1 : 0;
 }
 
 // Work items per cell that the streamed form takes: (binding chunks) x (parts of the cell's wave
@@ -2047,7 +2037,6 @@ struct StepPlan {
   };
   std::vector<StreamLaunch> slaunches;      // units of the streamed form, in launches of bounded lists
   DevBuf d_ts_counts, d_ts_starts, d_ts_parts, d_cellmat, d_ts_items, d_ts_num, d_light;
-  DevBuf d_ts_launches;  // the launches' TsLaunch addresses
   int64_t ts_items = 0;                     // upper bound of any streamed launch's work items
   size_t ts_lds = 0;
   int ts_blocks = 0;
@@ -2527,10 +2516,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     P.d_cellmat = DevBuf(ctx, (size_t)cells * max_lists * sizeof(int2));
     P.d_ts_starts = DevBuf(ctx, cells * 8);
     P.d_light = DevBuf(ctx, cells);
-    std::vector<const TsLaunch*> la;
-    for (const auto& SL : P.slaunches) la.push_back(SL.d_launch.as<TsLaunch>());
-    P.d_ts_launches = DevBuf(ctx, la.size() * sizeof(const TsLaunch*));
-    copy_h2d(ctx, P.d_ts_launches.get(), la.data(), la.size() * sizeof(const TsLaunch*));
+
     P.ts_items = bound;
     P.d_ts_items = DevBuf(ctx, bound * sizeof(DevItem));
     P.d_ts_num = DevBuf(ctx, 16);
@@ -2632,7 +2618,7 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
     if (!P.slaunches.empty()) {
       PhaseScope ps(ctx, "work_items");
       hipLaunchKernelGGL(k_ts_light, dim3((unsigned)ceil_div(cells_T, 256)), dim3(256), 0, ctx->stream,
-                         P.d_ts_launches.as<const TsLaunch*>(), (int)P.slaunches.size(), P.d_light.as<uint8_t>());
+                         P.slaunches[0].d_launch.as<TsLaunch>(), P.d_light.as<uint8_t>());
       QF_CHECK_HIP(hipGetLastError());
     }
     for (size_t li = 0; li < P.slaunches.size(); ++li) {
