@@ -2713,7 +2713,43 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
 struct GraphCache {
   std::vector<int64_t> key;
   std::unique_ptr<StepPlan> plan;
+  // The raw arguments of the qf_evaluate_literals call the graph last served (empty when it
+  // was made by another entry point): an identical call replays it before grouping literals.
+  std::string call;
 };
+
+// Replays a captured step on the private stream after the caller's queued work and copies its
+// counts out.
+static void replay_graph(Context* ctx, const GraphCache& gc, uint64_t* h_counts, int num_slots) {
+  QF_CHECK_HIP(hipEventRecord(ctx->graph_order_event, ctx->stream));
+  QF_CHECK_HIP(hipStreamWaitEvent(ctx->graph_stream, ctx->graph_order_event, 0));
+  QF_CHECK_HIP(hipGraphLaunch(gc.plan->graph, ctx->graph_stream));
+  QF_CHECK_HIP(hipStreamSynchronize(ctx->graph_stream));
+  std::memcpy(h_counts, gc.plan->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
+}
+
+// Appends the raw arguments of an evaluation call (sizes, options, column addresses, argument
+// types and literal bytes) to `out`; two calls with equal strings compute the same step.
+static void call_signature(std::string& out, const qf_table& binding, int64_t num_positive, const qf_table* relations,
+                           const int* arg_types, int num_relations, const qf_literal* literals, int num_literals,
+                           const qf_eval_options& o, const uint64_t* d_bits) {
+  auto put = [&](const void* p, size_t n) { out.append(static_cast<const char*>(p), n); };
+  const int64_t head[] = {binding.num_rows, binding.num_columns, num_positive, num_relations, num_literals,
+                          (int64_t)(uintptr_t)d_bits, arg_types ? 1 : 0};
+  out.clear();
+  put(head, sizeof(head));
+  put(&o, sizeof(o));
+  put(binding.columns, (size_t)binding.num_columns * sizeof(void*));
+  int64_t ntypes = 0;
+  for (int r = 0; r < num_relations; ++r) {
+    const int64_t shape[] = {relations[r].num_rows, relations[r].num_columns};
+    put(shape, sizeof(shape));
+    put(relations[r].columns, (size_t)relations[r].num_columns * sizeof(void*));
+    ntypes += relations[r].num_columns;
+  }
+  if (arg_types) put(arg_types, (size_t)ntypes * sizeof(int));
+  put(literals, (size_t)num_literals * sizeof(qf_literal));
+}
 
 static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_positive, const qf_table* relations,
                             std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o,
@@ -2819,14 +2855,9 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
       QF_CHECK_HIP(hipStreamCreateWithFlags(&ctx->graph_stream, hipStreamNonBlocking));
       QF_CHECK_HIP(hipEventCreateWithFlags(&ctx->graph_order_event, hipEventDisableTiming));
     }
-    // The graph runs on the private stream after everything already queued on the caller's.
-    QF_CHECK_HIP(hipEventRecord(ctx->graph_order_event, ctx->stream));
-    QF_CHECK_HIP(hipStreamWaitEvent(ctx->graph_stream, ctx->graph_order_event, 0));
     auto* gc = static_cast<GraphCache*>(ctx->graph_cache.get());
     if (gc && gc->key == key && gc->plan && gc->plan->graph) {
-      QF_CHECK_HIP(hipGraphLaunch(gc->plan->graph, ctx->graph_stream));
-      QF_CHECK_HIP(hipStreamSynchronize(ctx->graph_stream));
-      std::memcpy(h_counts, gc->plan->pinned, (size_t)num_slots * 4 * sizeof(uint64_t));
+      replay_graph(ctx, *gc, h_counts, num_slots);
       return;
     }
   }
@@ -2989,9 +3020,24 @@ int qf_evaluate_literals(qf_context* c, const qf_table* binding, int64_t num_pos
     qf_eval_options o;
     qf_eval_options_default(&o);
     if (opts) o = *opts;
+    // A call identical to the one the captured step last served replays it without regrouping
+    // the literals or rebuilding the step's key.
+    std::string call;
+    if (o.use_graph && o.reuse_partitions && o.mode == QF_EVAL_BATCHED && num_literals > 0) {
+      call_signature(call, *binding, num_positive, relations, arg_types, num_relations, literals, num_literals, o,
+                     d_semi_bits);
+      const auto* gc = static_cast<const GraphCache*>(ctx->graph_cache.get());
+      if (gc && gc->plan && gc->plan->graph && gc->call == call) {
+        replay_graph(ctx, *gc, h_counts, num_literals);
+        check_count_range(h_counts, num_literals);
+        return;
+      }
+    }
     std::vector<GroupSpec> specs =
         literal_groups(binding->num_columns, relations, arg_types, num_relations, literals, num_literals, o);
     run_groups(ctx, *binding, num_positive, relations, specs, num_literals, o, h_counts, d_semi_bits);
+    // The cached graph now computes this call's step (replayed or just captured).
+    if (!call.empty() && ctx->graph_cache) static_cast<GraphCache*>(ctx->graph_cache.get())->call = std::move(call);
     check_count_range(h_counts, num_literals);
   });
 }

@@ -215,6 +215,34 @@ def test_graph_replay_matches_eager(qf_ctx):
     qf_ctx.trim()
 
 
+def test_graph_replay_follows_changed_calls(qf_ctx):
+    """A repeated call replays the captured step straight from its raw arguments; any change of
+    literal list, num_positive or binding contents (same columns, rewritten in place) gives that
+    call's own counts, and returning to the first call gives the first counts again."""
+    binding, npos, rels, lits = random_problem(43, n=20000, rel_shapes=((2, 12000), (3, 7000)), domain=250,
+                                               zipf=True, max_literals=50)
+    dev_b = [_t(c) for c in binding]
+    dev_r = [[_t(c) for c in r] for r in rels]
+    kw = dict(use_graph=True, reuse_partitions=True)
+    want, _ = oracle.Problem(binding, npos, rels, lits).grouped()
+    for _ in range(2):
+        got, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, lits, **kw)
+        np.testing.assert_array_equal(got, want)
+    fewer = lits[: len(lits) // 2]
+    got, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, fewer, **kw)
+    np.testing.assert_array_equal(got, oracle.Problem(binding, npos, rels, fewer).grouped()[0])
+    got, _ = qf_ctx.evaluate_literals(dev_b, npos // 2, dev_r, lits, **kw)
+    np.testing.assert_array_equal(got, oracle.Problem(binding, npos // 2, rels, lits).grouped()[0])
+    got, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, lits, **kw)
+    np.testing.assert_array_equal(got, want)
+    shifted = [np.roll(c, 777) for c in binding]
+    for d, c in zip(dev_b, shifted):
+        d.copy_(_t(c))
+    got, _ = qf_ctx.evaluate_literals(dev_b, npos, dev_r, lits, **kw)
+    np.testing.assert_array_equal(got, oracle.Problem(shifted, npos, rels, lits).grouped()[0])
+    qf_ctx.trim()
+
+
 def test_autotuned_step_matches_golden(qf_ctx):
     """autotune=1 times its (facts per partition, tile) grid on the first call for a workload shape
     and runs the fastest; the counts of the 1/256 refinement step equal the committed golden ones,
