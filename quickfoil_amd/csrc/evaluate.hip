@@ -2025,7 +2025,8 @@ struct StepPlan {
   std::vector<int> join_vars;
   std::vector<std::vector<DevBuf>> tparts;  // per join variable: staged columns + row id
   std::vector<DevBuf> toffs;
-  std::vector<DevBuf> tscratch;             // per join variable: its partition's temporaries
+  std::vector<std::vector<int>> pgroups;    // join variables partitioned together (indices)
+  std::vector<DevBuf> tscratch;             // per group: its partition's temporaries
   std::vector<std::shared_ptr<RelPart>> keep_rel;
   std::vector<std::shared_ptr<HeavyTables>> keep_heavy;
   std::vector<DevUnit> units;
@@ -2104,15 +2105,24 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.join_vars.assign(vset.begin(), vset.end());
   QF_REQUIRE((int)P.bcols.size() <= EV_MAXTC, "too many binding columns referenced by one step");
   const int64_t rows = std::max<int64_t>(1, binding.num_rows);
-  // The binding partitions of the join variables are independent and run side by side, each on
-  // its own stream with its temporaries in a region of its own (a batch of at most 6 data
-  // columns plus the key and the row id is the widest pass).
-  const int part_cols = (int)std::min<size_t>(P.bcols.size(), 7) + 1;
+  // The binding partitions of the join variables: up to PT_MAXKEYS of them in one group whose
+  // first pass reads the binding columns once, when the staged columns and the row id fit the
+  // shared pass (six columns); otherwise one variable per group.  Groups are independent and run
+  // side by side, each on its own stream with its temporaries in a region of its own (a batch of
+  // at most 6 data columns plus the key and the row id is the widest single-variable pass).
   for (size_t v = 0; v < P.join_vars.size(); ++v) {
     P.tparts.emplace_back();
     for (size_t c = 0; c <= P.bcols.size(); ++c) P.tparts.back().emplace_back(ctx, rows * 4);
     P.toffs.emplace_back(ctx, (((size_t)1 << P.R_T) + 1) * 8);
-    P.tscratch.emplace_back(ctx, std::max<size_t>(256, partition_scratch_bytes(binding.num_rows, part_cols, P.R_T)));
+  }
+  const bool shared_pass = P.bcols.size() + 1 <= 6;
+  const int per_group = shared_pass ? PT_MAXKEYS : 1;
+  for (int v = 0; v < (int)P.join_vars.size(); v += per_group) {
+    P.pgroups.emplace_back();
+    for (int k = v; k < std::min<int>(v + per_group, (int)P.join_vars.size()); ++k) P.pgroups.back().push_back(k);
+    const int nk = (int)P.pgroups.back().size();
+    const int part_cols = shared_pass ? (int)P.bcols.size() + 1 : (int)std::min<size_t>(P.bcols.size(), 7) + 1;
+    P.tscratch.emplace_back(ctx, std::max<size_t>(256, partition_scratch_bytes(binding.num_rows, part_cols, P.R_T, nk)));
   }
   auto var_index = [&](int v) { return (int)(std::find(P.join_vars.begin(), P.join_vars.end(), v) - P.join_vars.begin()); };
   auto tidx = [&](int c) { return (int)(std::find(P.bcols.begin(), P.bcols.end(), c) - P.bcols.begin()); };
@@ -2585,31 +2595,48 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
   if (!P.units.empty() && binding.num_rows > 0) {
     {
       PhaseScope ps(ctx, "partition_binding");
-      const int nv = (int)P.join_vars.size();
-      ensure_aux_streams(ctx, nv);
+      const int ng = (int)P.pgroups.size();
+      ensure_aux_streams(ctx, ng);
       hipStream_t main = ctx->stream;
-      if (nv > 1) QF_CHECK_HIP(hipEventRecord(ctx->aux_events[0], main));
-      for (int v = 0; v < nv; ++v) {
-        // Join variable v on aux stream v - 1 (the first on the context stream), after all work
-        // already queued on the context stream.
-        hipStream_t st = v == 0 ? main : ctx->aux_streams[v - 1];
-        if (v > 0) QF_CHECK_HIP(hipStreamWaitEvent(st, ctx->aux_events[0], 0));
-        std::vector<int32_t*> outs;
-        for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
+      if (ng > 1) QF_CHECK_HIP(hipEventRecord(ctx->aux_events[0], main));
+      for (int g = 0; g < ng; ++g) {
+        // Group g on aux stream g - 1 (the first on the context stream), after all work already
+        // queued on the context stream.
+        hipStream_t st = g == 0 ? main : ctx->aux_streams[g - 1];
+        if (g > 0) QF_CHECK_HIP(hipStreamWaitEvent(st, ctx->aux_events[0], 0));
         Scratch scratch;
-        scratch.base = P.tscratch[v].as<char>();
-        scratch.capacity = P.tscratch[v].bytes();
+        scratch.base = P.tscratch[g].as<char>();
+        scratch.capacity = P.tscratch[g].bytes();
         ctx->stream = st;
         try {
-          partition_binding(ctx, binding, P.join_vars[v], P.bcols, P.R_T, outs, P.toffs[v].as<int64_t>(), &scratch);
+          const std::vector<int>& grp = P.pgroups[g];
+          if (grp.size() > 1) {
+            std::vector<const int32_t*> in;
+            for (int c : P.bcols) in.push_back(binding.columns[c]);
+            std::vector<int> keys;
+            std::vector<int32_t*> outs;
+            std::vector<int64_t*> offs;
+            for (int v : grp) {
+              keys.push_back((int)(std::find(P.bcols.begin(), P.bcols.end(), P.join_vars[v]) - P.bcols.begin()));
+              for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
+              offs.push_back(P.toffs[v].as<int64_t>());
+            }
+            stable_partition_multi(ctx, in.data(), (int)in.size(), keys.data(), (int)keys.size(), true,
+                                   binding.num_rows, 1, P.R_T, outs.data(), offs.data(), &scratch);
+          } else {
+            const int v = grp[0];
+            std::vector<int32_t*> outs;
+            for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
+            partition_binding(ctx, binding, P.join_vars[v], P.bcols, P.R_T, outs, P.toffs[v].as<int64_t>(), &scratch);
+          }
         } catch (...) {
           ctx->stream = main;
           throw;
         }
         ctx->stream = main;
-        if (v > 0) {
-          QF_CHECK_HIP(hipEventRecord(ctx->aux_events[v], st));
-          QF_CHECK_HIP(hipStreamWaitEvent(main, ctx->aux_events[v], 0));
+        if (g > 0) {
+          QF_CHECK_HIP(hipEventRecord(ctx->aux_events[g], st));
+          QF_CHECK_HIP(hipStreamWaitEvent(main, ctx->aux_events[g], 0));
         }
       }
     }

@@ -211,6 +211,11 @@ void scan_exclusive_i64(Context* ctx, const int64_t* in, int64_t* out, int64_t n
 
 // ------------------------------------------------------------------------------------------
 // Stable radix partition.
+//
+// One call partitions the same rows by up to PT_MAXKEYS key columns (the binding table by each
+// clause join variable): every key has its own outputs, bases and offsets.  The first pass reads
+// each input tile once and scatters it for every key in turn; later passes and the offsets run
+// over all keys' tiles in one launch.
 
 constexpr int PT_THREADS = 1024;
 constexpr int PT_WAVES = PT_THREADS / 64;
@@ -218,16 +223,19 @@ constexpr int PT_IPT = 4;
 constexpr int PT_TILE = PT_THREADS * PT_IPT;
 constexpr int PT_MAXCOLS = 8;
 constexpr int PT_MAXBINS = 256;
+static_assert(PT_MAXKEYS == 4, "key slots of PassArgs");
 
 struct PassArgs {
-  const int32_t* in[PT_MAXCOLS];
-  int32_t* out[PT_MAXCOLS];
+  const int32_t* in[PT_MAXKEYS][PT_MAXCOLS];  // per key: input columns (the same for every key in
+                                              // a first pass that reads each tile once)
+  int32_t* out[PT_MAXKEYS][PT_MAXCOLS];       // per key: output columns
+  int key_col[PT_MAXKEYS];
+  int nkeys;
   int ncols;      // columns written (inputs plus the generated row id, if any)
   int tid_col;    // index of the generated row-id column, or -1
-  int key_col;
   int out_stride;
-  int64_t n;
-  int64_t num_tiles;
+  int64_t n;          // rows of every key's input
+  int64_t num_tiles;  // tiles of every key's input
   int mixed, shift, bits;
 };
 
@@ -236,19 +244,21 @@ __device__ inline uint32_t digit_of(int32_t k, int mixed, int shift, int bits) {
   return bits == 0 ? 0u : (h >> shift) & ((1u << bits) - 1u);
 }
 
-// Digit counts of one tile per wave (eight tiles per workgroup), written bin-major
-// (counts[b * num_tiles + t]) so that one exclusive scan gives every tile its output base per bin.
+// Digit counts of one tile per wave (sixteen tiles per workgroup, key blockIdx.y), written
+// key-major, then bin-major (counts[(key * bins + b) * num_tiles + t]), so that one exclusive scan
+// gives every (key, tile) its output base per bin, plus key * n.
 constexpr int PT_HIST_RPL = PT_TILE / 64;  // rows per lane
 __global__ __launch_bounds__(PT_THREADS) void k_part_hist(PassArgs a, int64_t* counts) {
   __shared__ uint32_t hist[PT_WAVES][PT_MAXBINS];
   const int nb = 1 << a.bits;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int key_index = blockIdx.y;
   const int64_t tile = (int64_t)blockIdx.x * PT_WAVES + w;
   if (tile >= a.num_tiles) return;  // whole waves only; no workgroup barrier follows
   for (int b = lane; b < nb; b += 64) hist[w][b] = 0;
   __builtin_amdgcn_wave_barrier();
   const int64_t base = tile * PT_TILE;
-  const int32_t* key = a.in[a.key_col] + base;
+  const int32_t* key = a.in[key_index][a.key_col[key_index]] + base;
   const int rows = (int)min((int64_t)PT_TILE, a.n - base);
   int32_t k[PT_HIST_RPL];
   if (rows == PT_TILE) {
@@ -260,7 +270,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_hist(PassArgs a, int64_t* c
     for (int r = lane; r < rows; r += 64) atomicAdd(&hist[w][digit_of(key[r], a.mixed, a.shift, a.bits)], 1u);
   }
   __builtin_amdgcn_wave_barrier();
-  for (int b = lane; b < nb; b += 64) counts[(int64_t)b * a.num_tiles + tile] = hist[w][b];
+  int64_t* out = counts + (int64_t)key_index * nb * a.num_tiles;
+  for (int b = lane; b < nb; b += 64) out[(int64_t)b * a.num_tiles + tile] = hist[w][b];
 }
 
 // LDS of the scatter kernel beyond its static arrays: the staged tile (ncols x PT_TILE words)
@@ -289,10 +300,10 @@ struct TileWalk {
   }
 };
 
-// Loads one tile's rows (NC columns) into registers: every column's loads are in flight together.
-// Addresses are a wave-uniform base per column plus the lane's offset.
+// Loads one tile of key `ki`'s input (NC columns) into registers: every column's loads are in
+// flight together.  Addresses are a wave-uniform base per column plus the lane's offset.
 template <int NC>
-__device__ inline void load_tile(const PassArgs& a, int64_t tile, int lane, int w, int32_t (&v)[PT_IPT][NC]) {
+__device__ inline void load_tile(const PassArgs& a, int ki, int64_t tile, int lane, int w, int32_t (&v)[PT_IPT][NC]) {
   const int64_t wave_base = tile * PT_TILE + (int64_t)w * 64 * PT_IPT;
   if (wave_base + 64 * PT_IPT <= a.n) {
 #pragma unroll
@@ -301,7 +312,7 @@ __device__ inline void load_tile(const PassArgs& a, int64_t tile, int lane, int 
 #pragma unroll
         for (int i = 0; i < PT_IPT; ++i) v[i][c] = (int32_t)(wave_base + i * 64 + lane);
       } else {
-        const int32_t* src = a.in[c] + wave_base;
+        const int32_t* src = a.in[ki][c] + wave_base;
 #pragma unroll
         for (int i = 0; i < PT_IPT; ++i) v[i][c] = src[i * 64 + lane];
       }
@@ -313,15 +324,17 @@ __device__ inline void load_tile(const PassArgs& a, int64_t tile, int lane, int 
     const int64_t row = wave_base + (int64_t)i * 64 + lane;
     const bool ok = row < a.n;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[i][c] = c == a.tid_col ? (int32_t)row : (ok ? a.in[c][row] : 0);
+    for (int c = 0; c < NC; ++c) v[i][c] = c == a.tid_col ? (int32_t)row : (ok ? a.in[ki][c][row] : 0);
   }
 }
 
 // Persistent workgroups, software pipelined: a tile's rows are loaded into registers while the
-// previous tile is written out of LDS.
+// previous tile is written out of LDS.  SHARED (a first pass over one input for every key): a
+// workgroup walks the input's tiles and scatters each tile once per key; otherwise it walks the
+// tiles of all keys' inputs, key after key.
 // NC = a.ncols.  4096-row tiles: at 7-bit digits a bin's run in a tile averages 32 rows (128 B per
 // column), which measured 10-16 % faster than 2048-row tiles at the same number of waves per CU.
-template <int NC>
+template <int NC, bool SHARED>
 __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_part_scatter(
     PassArgs a, const int64_t* bases) {
   extern __shared__ int32_t stage[];                // ncols x PT_TILE rows in bin order, then digits
@@ -337,107 +350,127 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   const int w = threadIdx.x >> 6;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int wsh = 16 * (w & 1);
-  TileWalk walk(a.num_tiles);
+  // Walk position g -> (key of the input, tile of that input).
+  const int64_t T = a.num_tiles;
+  TileWalk walk(SHARED ? T : T * a.nkeys);
   int32_t v[PT_IPT][NC];
-  if (walk.next < walk.end) load_tile<NC>(a, walk.next, lane, w, v);
+  if (walk.next < walk.end)
+    load_tile<NC>(a, SHARED ? 0 : (int)(walk.next / T), SHARED ? walk.next : walk.next % T, lane, w, v);
 
-  for (int64_t tile = walk.next; tile < walk.end; tile += walk.step) {
+  for (int64_t g = walk.next; g < walk.end; g += walk.step) {
+    const int kin = SHARED ? 0 : (int)(g / T);
+    const int64_t tile = SHARED ? g : g % T;
+    const int k_end = SHARED ? a.nkeys : kin + 1;
     // Each wave owns PT_IPT consecutive 64-row groups of the tile: (wave, group, lane) is row order.
     const int64_t tile_base = tile * PT_TILE;
     const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
-    for (int b = threadIdx.x; b < PT_WAVES / 2 * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0])[b] = 0;
-    uint32_t d[PT_IPT];
+    for (int k = kin; k < k_end; ++k) {
+      const int key_col = a.key_col[k];
+      for (int b = threadIdx.x; b < PT_WAVES / 2 * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0])[b] = 0;
+      uint32_t d[PT_IPT];
 #pragma unroll
-    for (int i = 0; i < PT_IPT; ++i) {
-      int32_t key = v[i][0];
+      for (int i = 0; i < PT_IPT; ++i) {
+        int32_t key = v[i][0];
 #pragma unroll
-      for (int c = 1; c < NC; ++c)
-        if (c == a.key_col) key = v[i][c];
-      d[i] = digit_of(key, a.mixed, a.shift, a.bits);
-    }
-    __syncthreads();  // counts cleared; the previous tile's write-out is done with LDS
+        for (int c = 1; c < NC; ++c)
+          if (c == key_col) key = v[i][c];
+        d[i] = digit_of(key, a.mixed, a.shift, a.bits);
+      }
+      __syncthreads();  // counts cleared; the previous write-out is done with LDS
 #pragma unroll
-    for (int i = 0; i < PT_IPT; ++i)
-      if (wave_base + (int64_t)i * 64 + lane < a.n) atomicAdd(&wrun[w >> 1][d[i]], 1u << wsh);
-    __syncthreads();
+      for (int i = 0; i < PT_IPT; ++i)
+        if (wave_base + (int64_t)i * 64 + lane < a.n) atomicAdd(&wrun[w >> 1][d[i]], 1u << wsh);
+      __syncthreads();
 
-    // Bin totals, their exclusive scan (one lane per bin, 64 bins per wave), per-wave starts.
-    uint32_t tot = 0, incl = 0;
-    if (threadIdx.x < PT_MAXBINS) {
-      if ((int)threadIdx.x < nb)
+      // Bin totals, their exclusive scan (one lane per bin, 64 bins per wave), per-wave starts.
+      uint32_t tot = 0, incl = 0;
+      if (threadIdx.x < PT_MAXBINS) {
+        if ((int)threadIdx.x < nb)
+          for (int ww = 0; ww < PT_WAVES / 2; ++ww) {
+            const uint32_t pair = wrun[ww][threadIdx.x];
+            tot += (pair & 0xffffu) + (pair >> 16);
+          }
+        incl = tot;
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t y = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += y;
+        }
+        if (lane == 63) wtot[w] = incl;
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < nb) {
+        uint32_t excl = incl - tot;
+        for (int ww = 0; ww < w; ++ww) excl += wtot[ww];
+        uint32_t at = excl;
         for (int ww = 0; ww < PT_WAVES / 2; ++ww) {
           const uint32_t pair = wrun[ww][threadIdx.x];
-          tot += (pair & 0xffffu) + (pair >> 16);
+          const uint32_t lo = at, hi = at + (pair & 0xffffu);
+          wrun[ww][threadIdx.x] = lo | (hi << 16);
+          at = hi + (pair >> 16);
         }
-      incl = tot;
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
+        // The scan over all keys' counts puts key k's rows after the k * n rows of the keys before.
+        gbase[threadIdx.x] = bases[((int64_t)k * nb + threadIdx.x) * T + tile] - (int64_t)k * a.n - (int64_t)excl;
       }
-      if (lane == 63) wtot[w] = incl;
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nb) {
-      uint32_t excl = incl - tot;
-      for (int ww = 0; ww < w; ++ww) excl += wtot[ww];
-      uint32_t at = excl;
-      for (int ww = 0; ww < PT_WAVES / 2; ++ww) {
-        const uint32_t pair = wrun[ww][threadIdx.x];
-        const uint32_t lo = at, hi = at + (pair & 0xffffu);
-        wrun[ww][threadIdx.x] = lo | (hi << 16);
-        at = hi + (pair >> 16);
-      }
-      gbase[threadIdx.x] = bases[(int64_t)threadIdx.x * a.num_tiles + tile] - (int64_t)excl;
-    }
-    __syncthreads();
+      __syncthreads();
 
-    // Stable ranks within the wave (peers: lanes of the group with the same digit, one ballot per
-    // digit bit), rows staged in LDS in bin order.
+      // Stable ranks within the wave (peers: lanes of the group with the same digit, one ballot per
+      // digit bit), rows staged in LDS in bin order.
 #pragma unroll
-    for (int i = 0; i < PT_IPT; ++i) {
-      const int64_t row = wave_base + (int64_t)i * 64 + lane;
-      const bool valid = row < a.n;
-      const uint64_t vmask = __ballot(valid);
-      if (vmask == 0) break;
-      uint64_t peers = vmask;
-      for (int k = 0; k < a.bits; ++k) {
-        const bool bit = (d[i] >> k) & 1u;
-        const uint64_t bk = __ballot(bit);
-        peers &= bit ? bk : ~bk;
+      for (int i = 0; i < PT_IPT; ++i) {
+        const int64_t row = wave_base + (int64_t)i * 64 + lane;
+        const bool valid = row < a.n;
+        const uint64_t vmask = __ballot(valid);
+        if (vmask == 0) break;
+        uint64_t peers = vmask;
+        for (int kb = 0; kb < a.bits; ++kb) {
+          const bool bit = (d[i] >> kb) & 1u;
+          const uint64_t bk = __ballot(bit);
+          peers &= bit ? bk : ~bk;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        uint32_t pos = 0;
+        if (valid) pos = ((wrun[w >> 1][d[i]] >> wsh) & 0xffffu) + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) stage[c * PT_TILE + pos] = v[i][c];
+          dig8[pos] = (uint8_t)d[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) atomicAdd(&wrun[w >> 1][d[i]], (uint32_t)__popcll(peers) << wsh);
+        __builtin_amdgcn_wave_barrier();
       }
-      const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-      uint32_t pos = 0;
-      if (valid) pos = ((wrun[w >> 1][d[i]] >> wsh) & 0xffffu) + rank;
-      __builtin_amdgcn_wave_barrier();
-      if (valid) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) stage[c * PT_TILE + pos] = v[i][c];
-        dig8[pos] = (uint8_t)d[i];
+      __syncthreads();
+
+      // After the last key's staging the next tile's loads fly while this one is written out.
+      if (k == k_end - 1 && g + walk.step < walk.end) {
+        const int64_t gn = g + walk.step;
+        load_tile<NC>(a, SHARED ? 0 : (int)(gn / T), SHARED ? gn : gn % T, lane, w, v);
       }
-      __builtin_amdgcn_wave_barrier();
-      if (valid && rank == 0) atomicAdd(&wrun[w >> 1][d[i]], (uint32_t)__popcll(peers) << wsh);
-      __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
 
-    // The next tile's loads fly while this one is written out.
-    if (tile + walk.step < walk.end) load_tile<NC>(a, tile + walk.step, lane, w, v);
-
-    // Consecutive threads store consecutive rows of a bin.
-    const int tile_rows = (int)min((int64_t)PT_TILE, a.n - tile_base);
-    for (int j = threadIdx.x; j < tile_rows; j += PT_THREADS) {
-      const int64_t dest = gbase[dig8[j]] + j;
+      // Consecutive threads store consecutive rows of a bin.
+      const int tile_rows = (int)min((int64_t)PT_TILE, a.n - tile_base);
+      int32_t* const* out = a.out[k];
+      for (int j = threadIdx.x; j < tile_rows; j += PT_THREADS) {
+        const int64_t dest = gbase[dig8[j]] + j;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) a.out[c][dest * a.out_stride] = stage[c * PT_TILE + j];
+        for (int c = 0; c < NC; ++c) out[c][dest * a.out_stride] = stage[c * PT_TILE + j];
+      }
     }
   }
 }
 
 // offsets[b] = first row whose digit is >= b, for b in [0, num_parts]; rows are sorted by digit.
-// Each thread takes PT_OFF_RPT rows, 256 apart (coalesced).
+// Each thread takes PT_OFF_RPT rows, 256 apart (coalesced); key blockIdx.y.
 constexpr int PT_OFF_RPT = 8;
-__global__ void k_part_offsets(const int32_t* key, int stride, int64_t n, int mixed, int shift, int bits,
-                               int64_t* offsets, int64_t num_parts) {
+struct OffsetArgs {
+  const int32_t* key[PT_MAXKEYS];
+  int64_t* offsets[PT_MAXKEYS];
+};
+__global__ void k_part_offsets(OffsetArgs o, int stride, int64_t n, int mixed, int shift, int bits,
+                               int64_t num_parts) {
+  const int32_t* key = o.key[blockIdx.y];
+  int64_t* offsets = o.offsets[blockIdx.y];
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x * PT_OFF_RPT + threadIdx.x;
   if (n == 0) {
     if (i0 == 0)
@@ -464,31 +497,48 @@ __global__ void k_part_offsets(const int32_t* key, int stride, int64_t n, int mi
   }
 }
 
-template <int NC>
-static void launch_scatter(Context* ctx, const PassArgs& a, const int64_t* bases) {
+template <int NC, bool SHARED>
+static void launch_scatter_nc(Context* ctx, const PassArgs& a, const int64_t* bases) {
   const size_t lds = scatter_lds_bytes(NC);
   // Once per device (outside any graph capture in practice: the first pass runs eagerly).
   static bool lds_attr[64] = {};
   int dev = 0;
   QF_CHECK_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64 || !lds_attr[dev]) {
-    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter<NC>),
+    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter<NC, SHARED>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (dev >= 0 && dev < 64) lds_attr[dev] = true;
   }
   // Persistent grid: as many workgroups as fit the GPU at once (a multiple of 8), at most one per tile.
   int per_cu = 0;
-  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter<NC>, PT_THREADS, lds));
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter<NC, SHARED>, PT_THREADS, lds));
   int64_t grid = (int64_t)std::max(1, per_cu) * ctx->num_cus;
   if (grid >= 8) grid &= ~int64_t(7);
-  grid = std::min<int64_t>(grid, a.num_tiles);
-  hipLaunchKernelGGL(k_part_scatter<NC>, dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
+  grid = std::min<int64_t>(grid, SHARED ? a.num_tiles : a.num_tiles * a.nkeys);
+  hipLaunchKernelGGL((k_part_scatter<NC, SHARED>), dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
 }
 
-static void run_pass(Context* ctx, PassArgs a, Scratch* scratch) {
+template <bool SHARED>
+static void launch_scatter(Context* ctx, const PassArgs& a, const int64_t* bases) {
+  switch (a.ncols) {
+    case 1: launch_scatter_nc<1, SHARED>(ctx, a, bases); break;
+    case 2: launch_scatter_nc<2, SHARED>(ctx, a, bases); break;
+    case 3: launch_scatter_nc<3, SHARED>(ctx, a, bases); break;
+    case 4: launch_scatter_nc<4, SHARED>(ctx, a, bases); break;
+    case 5: launch_scatter_nc<5, SHARED>(ctx, a, bases); break;
+    case 6: launch_scatter_nc<6, SHARED>(ctx, a, bases); break;
+    case 7: launch_scatter_nc<7, SHARED>(ctx, a, bases); break;
+    default: launch_scatter_nc<8, SHARED>(ctx, a, bases); break;
+  }
+}
+
+// One pass: per-tile counts of every key, their exclusive scan, the scatter.  `shared`: every
+// key reads the same input (a.in[0]), each tile read once.
+static void run_pass(Context* ctx, PassArgs a, bool shared, Scratch* scratch) {
   a.num_tiles = std::max<int64_t>(1, ceil_div(a.n, PT_TILE));
   const int nb = 1 << a.bits;
-  const size_t count_bytes = (size_t)nb * a.num_tiles * sizeof(int64_t);
+  const int64_t m = (int64_t)a.nkeys * nb * a.num_tiles;
+  const size_t count_bytes = (size_t)m * sizeof(int64_t);
   DevBuf pooled;
   int64_t* counts = nullptr;
   if (scratch) {
@@ -497,20 +547,14 @@ static void run_pass(Context* ctx, PassArgs a, Scratch* scratch) {
     pooled = DevBuf(ctx, count_bytes);
     counts = pooled.as<int64_t>();
   }
-  hipLaunchKernelGGL(k_part_hist, dim3((unsigned)ceil_div(a.num_tiles, PT_WAVES)), dim3(PT_THREADS), 0,
-                     ctx->stream, a, counts);
+  hipLaunchKernelGGL(k_part_hist, dim3((unsigned)ceil_div(a.num_tiles, PT_WAVES), (unsigned)a.nkeys), dim3(PT_THREADS),
+                     0, ctx->stream, a, counts);
   QF_CHECK_HIP(hipGetLastError());
-  scan_exclusive_i64(ctx, counts, counts, (int64_t)nb * a.num_tiles, scratch);
-  switch (a.ncols) {
-    case 1: launch_scatter<1>(ctx, a, counts); break;
-    case 2: launch_scatter<2>(ctx, a, counts); break;
-    case 3: launch_scatter<3>(ctx, a, counts); break;
-    case 4: launch_scatter<4>(ctx, a, counts); break;
-    case 5: launch_scatter<5>(ctx, a, counts); break;
-    case 6: launch_scatter<6>(ctx, a, counts); break;
-    case 7: launch_scatter<7>(ctx, a, counts); break;
-    default: launch_scatter<8>(ctx, a, counts); break;
-  }
+  scan_exclusive_i64(ctx, counts, counts, m, scratch);
+  if (shared && a.nkeys > 1)
+    launch_scatter<true>(ctx, a, counts);
+  else
+    launch_scatter<false>(ctx, a, counts);
   QF_CHECK_HIP(hipGetLastError());
 }
 
@@ -522,36 +566,39 @@ static std::vector<int> pass_bits(int total_bits) {
   return bits;
 }
 
-size_t partition_scratch_bytes(int64_t num_rows, int ncols, int total_bits) {
+size_t partition_scratch_bytes(int64_t num_rows, int ncols, int total_bits, int nkeys) {
   if (num_rows <= 0) return 0;
   const std::vector<int> bits = pass_bits(total_bits);
   const int passes = (int)bits.size();
   size_t total = 0;
-  for (int s = 0; s < 2 && s < passes - 1; ++s) total += (size_t)ncols * aligned256((size_t)num_rows * sizeof(int32_t));
+  for (int s = 0; s < 2 && s < passes - 1; ++s)
+    total += (size_t)nkeys * ncols * aligned256((size_t)num_rows * sizeof(int32_t));
   const int64_t tiles = std::max<int64_t>(1, ceil_div(num_rows, PT_TILE));
   for (int b : bits) {
-    const int64_t m = ((int64_t)1 << b) * tiles;
+    const int64_t m = (int64_t)nkeys * ((int64_t)1 << b) * tiles;
     total += aligned256((size_t)m * sizeof(int64_t)) + scan_scratch_bytes(m);
   }
   return total;
 }
 
-static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col,
-                           bool gen_tid, int64_t num_rows, int mixed, int total_bits,
-                           int32_t* const* out_cols, int out_stride, int64_t* d_offsets, Scratch* scratch) {
+// out_cols[k * ncols + c]: output column c of key k; d_offsets[k] (or null): its partition starts.
+static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_in_cols, const int* key_cols,
+                           int nkeys, bool gen_tid, int64_t num_rows, int mixed, int total_bits,
+                           int32_t* const* out_cols, int out_stride, int64_t* const* d_offsets, Scratch* scratch) {
   const int ncols = num_in_cols + (gen_tid ? 1 : 0);
   QF_REQUIRE(ncols <= PT_MAXCOLS, "too many columns to partition");
+  QF_REQUIRE(nkeys >= 1 && nkeys <= PT_MAXKEYS, "bad number of partition keys");
   QF_REQUIRE(total_bits >= 0 && total_bits <= 32, "bad radix bit count");
-  QF_REQUIRE(key_col >= 0 && key_col < num_in_cols, "bad key column");
+  for (int k = 0; k < nkeys; ++k) QF_REQUIRE(key_cols[k] >= 0 && key_cols[k] < num_in_cols, "bad key column");
   if (num_rows > 0) {
     const std::vector<int> bits = pass_bits(total_bits);
     const int passes = (int)bits.size();
-    // Intermediate columns of the LSD passes (two sets, used alternately).
+    // Intermediate columns of the LSD passes (two sets, used alternately), per key.
     std::vector<DevBuf> pooled;
     std::vector<int32_t*> tmp[2];
     if (passes > 1)
       for (int s = 0; s < 2 && s < passes - 1; ++s)
-        for (int c = 0; c < ncols; ++c) {
+        for (int i = 0; i < nkeys * ncols; ++i) {
           const size_t bytes = (size_t)num_rows * sizeof(int32_t);
           if (scratch) {
             tmp[s].push_back(static_cast<int32_t*>(scratch->take(bytes)));
@@ -564,28 +611,25 @@ static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_
     for (int p = 0; p < passes; ++p) {
       PassArgs a{};
       a.n = num_rows;
-      a.key_col = key_col;
+      a.nkeys = nkeys;
       a.ncols = ncols;
       a.mixed = mixed;
       a.bits = bits[p];
       a.shift = mixed ? (32 - total_bits + lo) : lo;
       if (a.bits == 0) a.shift = 0;
-      if (p == 0) {
-        for (int c = 0; c < num_in_cols; ++c) a.in[c] = in_cols[c];
-        a.tid_col = gen_tid ? num_in_cols : -1;
-        if (gen_tid) a.in[num_in_cols] = nullptr;
-      } else {
-        for (int c = 0; c < ncols; ++c) a.in[c] = tmp[(p - 1) & 1][c];
-        a.tid_col = -1;
+      for (int k = 0; k < nkeys; ++k) {
+        a.key_col[k] = key_cols[k];
+        for (int c = 0; c < ncols; ++c) {
+          if (p == 0)
+            a.in[k][c] = c < num_in_cols ? in_cols[c] : nullptr;
+          else
+            a.in[k][c] = tmp[(p - 1) & 1][k * ncols + c];
+          a.out[k][c] = p == passes - 1 ? out_cols[k * ncols + c] : tmp[p & 1][k * ncols + c];
+        }
       }
-      if (p == passes - 1) {
-        for (int c = 0; c < ncols; ++c) a.out[c] = out_cols[c];
-        a.out_stride = out_stride;
-      } else {
-        for (int c = 0; c < ncols; ++c) a.out[c] = tmp[p & 1][c];
-        a.out_stride = 1;
-      }
-      run_pass(ctx, a, scratch);
+      a.tid_col = p == 0 && gen_tid ? num_in_cols : -1;
+      a.out_stride = p == passes - 1 ? out_stride : 1;
+      run_pass(ctx, a, p == 0, scratch);
       lo += bits[p];
     }
   }
@@ -593,9 +637,13 @@ static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_
     const int64_t parts = (int64_t)1 << total_bits;
     const int full_shift = mixed ? 32 - total_bits : 0;
     const int64_t blocks = std::max<int64_t>(1, ceil_div(num_rows, 256 * PT_OFF_RPT));
-    hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, out_cols[key_col],
-                       out_stride, num_rows, mixed, total_bits == 0 ? 0 : full_shift, total_bits, d_offsets,
-                       parts);
+    OffsetArgs o{};
+    for (int k = 0; k < nkeys; ++k) {
+      o.key[k] = out_cols[k * ncols + key_cols[k]];
+      o.offsets[k] = d_offsets[k];
+    }
+    hipLaunchKernelGGL(k_part_offsets, dim3((unsigned)blocks, (unsigned)nkeys), dim3(256), 0, ctx->stream, o,
+                       out_stride, num_rows, mixed, total_bits == 0 ? 0 : full_shift, total_bits, parts);
     QF_CHECK_HIP(hipGetLastError());
   }
 }
@@ -603,13 +651,22 @@ static void partition_impl(Context* ctx, const int32_t* const* in_cols, int num_
 void stable_partition(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col, bool gen_tid,
                       int64_t num_rows, int mixed, int total_bits, int32_t* const* out_cols, int out_stride,
                       int64_t* d_offsets, Scratch* scratch) {
-  partition_impl(ctx, in_cols, num_in_cols, key_col, gen_tid, num_rows, mixed, total_bits, out_cols, out_stride,
+  int64_t* offs[1] = {d_offsets};
+  partition_impl(ctx, in_cols, num_in_cols, &key_col, 1, gen_tid, num_rows, mixed, total_bits, out_cols, out_stride,
+                 d_offsets ? offs : nullptr, scratch);
+}
+
+void stable_partition_multi(Context* ctx, const int32_t* const* in_cols, int num_in_cols, const int* key_cols,
+                            int nkeys, bool gen_tid, int64_t num_rows, int mixed, int total_bits,
+                            int32_t* const* out_cols, int64_t* const* d_offsets, Scratch* scratch) {
+  partition_impl(ctx, in_cols, num_in_cols, key_cols, nkeys, gen_tid, num_rows, mixed, total_bits, out_cols, 1,
                  d_offsets, scratch);
 }
 
 void stable_sort_by_key(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col, bool gen_tid,
                         int64_t num_rows, int key_bits, int32_t* const* out_cols) {
-  partition_impl(ctx, in_cols, num_in_cols, key_col, gen_tid, num_rows, 0, key_bits, out_cols, 1, nullptr, nullptr);
+  partition_impl(ctx, in_cols, num_in_cols, &key_col, 1, gen_tid, num_rows, 0, key_bits, out_cols, 1, nullptr,
+                 nullptr);
 }
 
 }  // namespace qf

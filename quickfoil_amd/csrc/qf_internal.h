@@ -215,8 +215,18 @@ struct DigitFn {
 void stable_partition(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col,
                       bool gen_tid, int64_t num_rows, int mixed, int total_bits,
                       int32_t* const* out_cols, int out_stride, int64_t* d_offsets, Scratch* scratch = nullptr);
-// Bytes stable_partition takes from a Scratch for num_rows rows of ncols written columns.
-size_t partition_scratch_bytes(int64_t num_rows, int ncols, int total_bits);
+// Bytes stable_partition (nkeys = 1) or stable_partition_multi takes from a Scratch for num_rows
+// rows of ncols written columns per key.
+size_t partition_scratch_bytes(int64_t num_rows, int ncols, int total_bits, int nkeys = 1);
+
+// The same rows partitioned by each of nkeys <= PT_MAXKEYS key columns (mixed digits as
+// stable_partition, SoA outputs): out_cols[k * ncols + c] is written column c of key k, where
+// ncols = num_in_cols (+ 1 for the row id), and d_offsets[k] (or d_offsets null) receives key k's
+// 2^total_bits + 1 partition starts.  The first pass reads every input tile once for all keys.
+constexpr int PT_MAXKEYS = 4;
+void stable_partition_multi(Context* ctx, const int32_t* const* in_cols, int num_in_cols, const int* key_cols,
+                            int nkeys, bool gen_tid, int64_t num_rows, int mixed, int total_bits,
+                            int32_t* const* out_cols, int64_t* const* d_offsets, Scratch* scratch = nullptr);
 
 // Stable sort of (key, payload...) rows by uint32 key values below 2^key_bits (raw digits).
 void stable_sort_by_key(Context* ctx, const int32_t* const* in_cols, int num_in_cols, int key_col,
