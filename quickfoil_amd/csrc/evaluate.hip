@@ -435,19 +435,15 @@ __global__ void k_ts_light(const TsLaunch* L_, uint8_t* light) {
   light[q] = chunks <= TS_CMAX ? 1 : 0;
 }
 
-// Work items per cell that the streamed form takes: (binding chunks) x (parts of the cell's wave
-// chunks: its runs, all run lists one after another in rows of TS_RUNROW, TS_RUNROWS rows per
-// chunk, then its tuples in chunks of TS_TUPCH).  cellmat[q * nlists + l] = (first entry,
-// entries) of list l in cell q, so that a work item reads its cell's lists in one pass.
-__global__ void k_ts_count(const TsLaunch* L_, const uint8_t* light, int64_t* counts, int32_t* parts, int2* cellmat) {
+// The relation side of a streamed launch's cells, fixed by its key-tuple lists (made once per
+// plan): cellmat[q * nlists + l] = (first entry, entries) of list l in cell q, so that a work item
+// reads its cell's lists in one pass, and parts[q] = the parts of the cell's wave chunks (its
+// runs, all run lists one after another in rows of TS_RUNROW, TS_RUNROWS rows per chunk, then its
+// tuples in chunks of TS_TUPCH) in TS_PART chunks each.
+__global__ void k_ts_cells(const TsLaunch* L_, int32_t* parts, int2* cellmat) {
   cptr<TsLaunch> L = as_const(L_);
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= L->cells) return;
-  int64_t chunks = 0;
-  for (int v = 0; v < L->nvars; ++v) {
-    const int64_t rows = L->toff[v][q + 1] - L->toff[v][q];
-    chunks = max(chunks, (rows + L->tr - 1) / L->tr);
-  }
   cptr<TsList> lists = as_const(L->lists);
   int64_t rows = 0, nt = 0;
   for (int l = 0; l < L->nlists; ++l) {
@@ -457,9 +453,20 @@ __global__ void k_ts_count(const TsLaunch* L_, const uint8_t* light, int64_t* co
     else nt += n;
   }
   const int64_t wc = (rows + TS_RUNROWS - 1) / TS_RUNROWS + (nt + TS_TUPCH - 1) / TS_TUPCH;
-  const int64_t p = (wc + TS_PART - 1) / TS_PART;
-  parts[q] = (int32_t)max<int64_t>(1, p);
-  counts[q] = light[q] ? chunks * p : 0;
+  parts[q] = (int32_t)max<int64_t>(1, (wc + TS_PART - 1) / TS_PART);
+}
+
+// Work items per cell that the streamed form takes: (binding chunks) x (parts of the cell).
+__global__ void k_ts_count(const TsLaunch* L_, const uint8_t* light, const int32_t* parts, int64_t* counts) {
+  cptr<TsLaunch> L = as_const(L_);
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L->cells) return;
+  int64_t chunks = 0;
+  for (int v = 0; v < L->nvars; ++v) {
+    const int64_t rows = L->toff[v][q + 1] - L->toff[v][q];
+    chunks = max(chunks, (rows + L->tr - 1) / L->tr);
+  }
+  counts[q] = light[q] ? chunks * parts[q] : 0;
 }
 
 // One thread per work item: item i of cell q (the last cell whose scanned start is <= i) is
@@ -2034,10 +2041,11 @@ struct StepPlan {
   std::vector<std::shared_ptr<TupleList>> keep_tuples;
   struct StreamLaunch {
     DevBuf d_launch, d_lists, d_targets;
+    DevBuf d_parts, d_cellmat;  // per cell: parts; per cell and list: (first entry, entries)
     int64_t items_bound = 0;
   };
   std::vector<StreamLaunch> slaunches;      // units of the streamed form, in launches of bounded lists
-  DevBuf d_ts_counts, d_ts_starts, d_ts_parts, d_cellmat, d_ts_items, d_ts_num, d_light;
+  DevBuf d_ts_counts, d_ts_starts, d_ts_items, d_ts_num, d_light;
   int64_t ts_items = 0;                     // upper bound of any streamed launch's work items
   size_t ts_lds = 0;
   int ts_blocks = 0;
@@ -2511,6 +2519,12 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       tl.cells = cells;
       SL.d_launch = DevBuf(ctx, sizeof(TsLaunch));
       copy_h2d(ctx, SL.d_launch.get(), &tl, sizeof(TsLaunch));
+      // The cells' list ranges depend on the relations' key-tuple lists only: made once here.
+      SL.d_parts = DevBuf(ctx, cells * 4);
+      SL.d_cellmat = DevBuf(ctx, (size_t)cells * std::max<size_t>(1, pk.lists.size()) * sizeof(int2));
+      hipLaunchKernelGGL(k_ts_cells, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, ctx->stream,
+                         SL.d_launch.as<TsLaunch>(), SL.d_parts.as<int32_t>(), SL.d_cellmat.as<int2>());
+      QF_CHECK_HIP(hipGetLastError());
       // Items: at most TS_CMAX binding chunks per cell times its parts, which number at most
       // one per cell with entries plus the cell's wave chunks over TS_PART (a cell's chunks are
       // at most its entries / TS_RUNROW, plus one row per run list, over TS_RUNROWS, plus two).
@@ -2519,11 +2533,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       bound = std::max(bound, SL.items_bound);
       P.slaunches.push_back(std::move(SL));
     }
-    size_t max_lists = 1;
-    for (const Pack& pk : packs) max_lists = std::max(max_lists, pk.lists.size());
     P.d_ts_counts = DevBuf(ctx, cells * 8);
-    P.d_ts_parts = DevBuf(ctx, cells * 4);
-    P.d_cellmat = DevBuf(ctx, (size_t)cells * max_lists * sizeof(int2));
     P.d_ts_starts = DevBuf(ctx, cells * 8);
     P.d_light = DevBuf(ctx, cells);
 
@@ -2653,18 +2663,18 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
       {
         PhaseScope ps(ctx, "work_items");
         hipLaunchKernelGGL(k_ts_count, dim3((unsigned)ceil_div(cells_T, 256)), dim3(256), 0, ctx->stream,
-                           SL.d_launch.as<TsLaunch>(), P.d_light.as<uint8_t>(), P.d_ts_counts.as<int64_t>(),
-                           P.d_ts_parts.as<int32_t>(), P.d_cellmat.as<int2>());
+                           SL.d_launch.as<TsLaunch>(), P.d_light.as<uint8_t>(), SL.d_parts.as<int32_t>(),
+                           P.d_ts_counts.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), cells_T);
         hipLaunchKernelGGL(k_ts_expand, dim3((unsigned)ceil_div(SL.items_bound, 256)), dim3(256), 0, ctx->stream,
-                           P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), P.d_ts_parts.as<int32_t>(), cells_T,
+                           P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), SL.d_parts.as<int32_t>(), cells_T,
                            P.d_ts_items.as<DevItem>(), P.d_ts_num.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
       }
       TsArgs ta{};
       ta.launch = SL.d_launch.as<TsLaunch>();
-      ta.cellmat = P.d_cellmat.as<int2>();
+      ta.cellmat = SL.d_cellmat.as<int2>();
       ta.units = P.d_units.as<DevUnit>();
       ta.lits = P.d_lits.as<DevLit>();
       ta.items = P.d_ts_items.as<DevItem>();
