@@ -76,8 +76,9 @@ def algorithmic_partition_bytes(workload):
 
 
 def measured_traffic(workload, kernel="k_stream"):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    of this workload (profiles/<kernel>_traffic.json), or None when none was recorded."""
+    """HBM bytes per launch of `kernel` (per step for "binding_partition") from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (profiles/<kernel>_traffic.json), or
+    None when none was recorded."""
     path = os.path.join(ROOT, "profiles", "%s_traffic.json" % kernel)
     try:
         with open(path) as f:
@@ -401,6 +402,18 @@ def main():
         roofline["traffic_GBs"] = traffic / (probe_ms_per_step * 1e-3) / 1e9 if probe_ms_per_step > 0 else 0.0
         roofline["traffic_frac"] = roofline["traffic_GBs"] / HBM_PEAK_GBS
 
+    part_traffic = measured_traffic(args.workload, kernel="binding_partition")
+    roofline_part = {"bound": "hbm", "achieved": part_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": part_achieved / HBM_PEAK_GBS, "traffic": part_traffic,
+                     "kernel": "binding radix partition of the clause join variables (histograms, scans, scatter "
+                               "passes, offsets; HIP events around the phase)",
+                     "algorithmic_bytes_per_step": part_bytes}
+    if part_traffic:
+        # rocprofv3 FETCH_SIZE + WRITE_SIZE summed over the partition's kernels of one step: both LSD
+        # passes and their histograms, against the formula's single read and write per variable.
+        roofline_part["traffic_GBs"] = part_traffic / (part_ms_per_step * 1e-3) / 1e9 if part_ms_per_step > 0 else 0.0
+        roofline_part["traffic_frac"] = roofline_part["traffic_GBs"] / HBM_PEAK_GBS
+
     cpu = None
     if not args.no_cpu:
         fracs = args.cpu_fractions or ([1.0 / 256, 1.0 / 64] if args.workload == "step" else [1.0])
@@ -439,11 +452,7 @@ def main():
         "phase_ms_per_step": {k: v[0] / max(1, args.steps) for k, v in phases.items()},
         "roofline": roofline,
         "probe_phase_ms_per_step": probe_ms_per_step,
-        "roofline_partition": {"bound": "hbm", "achieved": part_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": part_achieved / HBM_PEAK_GBS, "traffic": None,
-                               "kernel": "binding radix partition per join variable (histogram, scan, scatter, "
-                                         "offsets; HIP events around the phase)",
-                               "algorithmic_bytes_per_step": part_bytes},
+        "roofline_partition": roofline_part,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
