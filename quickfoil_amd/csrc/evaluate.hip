@@ -1769,12 +1769,20 @@ static int literal_join_key(const qf_literal& l, int arity, const int* types) {
 // Sorts key positions by relation column; a relation column bound twice (P[c] == T[x] and
 // P[c] == T[y]) keeps one key position and adds the row-side equality T[x] == T[y].
 static void normalise_key(LitSpec& ls) {
-  std::vector<std::pair<int, int>> kv;
-  for (size_t i = 0; i < ls.pcols.size(); ++i) kv.emplace_back(ls.pcols[i], ls.bcols[i]);
-  std::sort(kv.begin(), kv.end());
+  const size_t n = ls.pcols.size();
+  std::pair<int, int> small[16];
+  std::vector<std::pair<int, int>> large;
+  std::pair<int, int>* kv = small;
+  if (n > 16) {
+    large.resize(n);
+    kv = large.data();
+  }
+  for (size_t i = 0; i < n; ++i) kv[i] = std::make_pair(ls.pcols[i], ls.bcols[i]);
+  std::sort(kv, kv + n);
   ls.pcols.clear();
   ls.bcols.clear();
-  for (auto& p : kv) {
+  for (size_t i = 0; i < n; ++i) {
+    const std::pair<int, int>& p = kv[i];
     if (!ls.pcols.empty() && ls.pcols.back() == p.first) {
       if (ls.bcols.back() != p.second) ls.eqs.emplace_back(ls.bcols.back(), p.second);
       continue;
@@ -2024,6 +2032,90 @@ static void partition_binding(Context* ctx, const qf_table& b, int var, const st
   } while (pos < others.size());
 }
 
+// Pinned host staging of a context.  A plan's small uploads (descriptors, lists, targets) are
+// copied into it and sent with asynchronous copies, without a stream synchronisation each; the
+// next plan waits for them (an event) before reusing the space.  An upload that does not fit goes
+// synchronously, and the region grows before the next plan.  `counts` receives the counts of
+// steps run without a graph (a captured step keeps its own).
+struct HostStage {
+  char* base = nullptr;
+  size_t cap = 0, used = 0, want = 0;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+  uint64_t* counts = nullptr;
+  size_t counts_cap = 0;
+  ~HostStage() {
+    if (pending && done) (void)hipEventSynchronize(done);
+    if (done) (void)hipEventDestroy(done);
+    if (base) (void)hipHostFree(base);
+    if (counts) (void)hipHostFree(counts);
+  }
+};
+
+static HostStage& host_stage(Context* ctx) {
+  if (!ctx->host_stage) ctx->host_stage = std::make_shared<HostStage>();
+  return *std::static_pointer_cast<HostStage>(ctx->host_stage);
+}
+
+// Starts a plan's uploads: the previous plan's copies are done with the region.
+static void begin_uploads(Context* ctx) {
+  HostStage& H = host_stage(ctx);
+  if (H.pending) {
+    QF_CHECK_HIP(hipEventSynchronize(H.done));
+    H.pending = false;
+  } else if (H.used > 0) {
+    sync(ctx);  // a plan that stopped early left copies without the event
+  }
+  if (!H.done) QF_CHECK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
+  if (H.want > H.cap) {
+    if (H.base) QF_CHECK_HIP(hipHostFree(H.base));
+    H.base = nullptr;
+    H.cap = 0;
+    const size_t cap = std::max<size_t>(H.want * 2, (size_t)1 << 20);
+    QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&H.base), cap, hipHostMallocDefault));
+    H.cap = cap;
+  }
+  H.used = 0;
+  H.want = 0;
+}
+
+// Host-to-device copy of a plan upload on the context stream (see HostStage).
+static void upload(Context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  HostStage& H = host_stage(ctx);
+  const size_t at = (H.used + 255) & ~(size_t)255;
+  if (H.base && at + bytes <= H.cap) {
+    std::memcpy(H.base + at, src, bytes);
+    QF_CHECK_HIP(hipMemcpyAsync(dst, H.base + at, bytes, hipMemcpyHostToDevice, ctx->stream));
+    H.used = at + bytes;
+  } else {
+    H.want = std::max(H.want, at + bytes);
+    copy_h2d(ctx, dst, src, bytes);
+  }
+}
+
+// Ends a plan's uploads: the event marks when the region may be reused.
+static void end_uploads(Context* ctx) {
+  HostStage& H = host_stage(ctx);
+  if (H.used > 0) {
+    QF_CHECK_HIP(hipEventRecord(H.done, ctx->stream));
+    H.pending = true;
+  }
+}
+
+// Pinned counts for a step run without a graph (the call synchronises before it returns).
+static uint64_t* stage_counts(Context* ctx, size_t bytes) {
+  HostStage& H = host_stage(ctx);
+  if (bytes > H.counts_cap) {
+    if (H.counts) QF_CHECK_HIP(hipHostFree(H.counts));
+    H.counts = nullptr;
+    H.counts_cap = 0;
+    QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&H.counts), bytes, hipHostMallocDefault));
+    H.counts_cap = bytes;
+  }
+  return H.counts;
+}
+
 // Everything one evaluation call needs on the device; kept alive by a cached graph.
 struct StepPlan {
   int R_T = 0;
@@ -2065,11 +2157,12 @@ struct StepPlan {
   int num_slots = 0;
   int64_t bit_words = 0;
   uint64_t* pinned = nullptr;
+  bool own_pinned = false;   // a captured step's own pinned counts (else the context's)
   hipGraphExec_t graph = nullptr;
   std::vector<DevBuf> held;  // pool blocks a captured graph uses
   ~StepPlan() {
     if (graph) (void)hipGraphExecDestroy(graph);
-    if (pinned) (void)hipHostFree(pinned);
+    if (pinned && own_pinned) (void)hipHostFree(pinned);
   }
 };
 
@@ -2081,6 +2174,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
                       const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P,
                       bool stream) {
   const bool cache = o.reuse_partitions != 0;
+  begin_uploads(ctx);
   P.num_slots = num_slots;
   P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
   P.tile = (P.tile + 63) & ~63;  // whole 64-row blocks
@@ -2200,15 +2294,22 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     // it); the others follow by key width, then first use.  Literals with row-side equalities
     // get subsets of their own.
     using SubKey = std::pair<std::vector<int>, bool>;
-    auto sub_key = [](const LitSpec* l) { return SubKey(l->pcols, !l->eqs.empty()); };
+    auto sub_of = [](const std::vector<SubKey>& subs, const LitSpec* l) {
+      for (size_t i = 0; i < subs.size(); ++i)
+        if (subs[i].second == !l->eqs.empty() && subs[i].first == l->pcols) return (int)i;
+      return -1;
+    };
     const int jc = S.key.second;
     std::vector<SubKey> subsets{SubKey(std::vector<int>{jc}, false)};
     for (int gi = 0; gi < du.ngroups; ++gi)
       for (const LitSpec* l : S.lits[gi])
-        if (std::find(subsets.begin(), subsets.end(), sub_key(l)) == subsets.end()) subsets.push_back(sub_key(l));
+        if (sub_of(subsets, l) < 0) subsets.emplace_back(l->pcols, !l->eqs.empty());
     std::stable_sort(subsets.begin() + 1, subsets.end(),
                      [](const SubKey& x, const SubKey& y) { return x.first.size() < y.first.size(); });
     QF_REQUIRE((int)subsets.size() <= EV_MAXSUB, "too many key subsets in one unit");
+    std::vector<std::vector<int>> lsub(du.ngroups);  // each literal's subset
+    for (int gi = 0; gi < du.ngroups; ++gi)
+      for (const LitSpec* l : S.lits[gi]) lsub[gi].push_back(sub_of(subsets, l));
     du.jcol = jc;
     int unit_lits = 0, nmask = 0;
     const size_t unit_first = P.lits.size();
@@ -2222,8 +2323,9 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       const int first = (int)P.lits.size();
       for (int s = 0; s < (int)subsets.size(); ++s) {
         G.sbeg[s] = (int)P.lits.size();
-        for (const LitSpec* l : S.lits[gi]) {
-          if (sub_key(l) != subsets[s]) continue;
+        for (size_t li = 0; li < S.lits[gi].size(); ++li) {
+          if (lsub[gi][li] != s) continue;
+          const LitSpec* l = S.lits[gi][li];
           DevLit dl{};
           dl.slot = l->slot;
           dl.subset = s;
@@ -2497,10 +2599,10 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     for (const Pack& pk : packs) {
       StepPlan::StreamLaunch SL;
       SL.d_lists = DevBuf(ctx, std::max<size_t>(1, pk.lists.size()) * sizeof(TsList));
-      if (!pk.lists.empty()) copy_h2d(ctx, SL.d_lists.get(), pk.lists.data(), pk.lists.size() * sizeof(TsList));
+      if (!pk.lists.empty()) upload(ctx, SL.d_lists.get(), pk.lists.data(), pk.lists.size() * sizeof(TsList));
       SL.d_targets = DevBuf(ctx, std::max<size_t>(1, pk.targets.size()) * sizeof(TsTarget));
       if (!pk.targets.empty())
-        copy_h2d(ctx, SL.d_targets.get(), pk.targets.data(), pk.targets.size() * sizeof(TsTarget));
+        upload(ctx, SL.d_targets.get(), pk.targets.data(), pk.targets.size() * sizeof(TsTarget));
       TsLaunch tl{};
       tl.lists = SL.d_lists.as<TsList>();
       tl.nlists = (int)pk.lists.size();
@@ -2518,7 +2620,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
       }
       tl.cells = cells;
       SL.d_launch = DevBuf(ctx, sizeof(TsLaunch));
-      copy_h2d(ctx, SL.d_launch.get(), &tl, sizeof(TsLaunch));
+      upload(ctx, SL.d_launch.get(), &tl, sizeof(TsLaunch));
       // The cells' list ranges depend on the relations' key-tuple lists only: made once here.
       SL.d_parts = DevBuf(ctx, cells * 4);
       SL.d_cellmat = DevBuf(ctx, (size_t)cells * std::max<size_t>(1, pk.lists.size()) * sizeof(int2));
@@ -2549,12 +2651,12 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   }
   P.d_units = DevBuf(ctx, std::max<size_t>(1, P.units.size()) * sizeof(DevUnit));
   P.d_batches = DevBuf(ctx, std::max<size_t>(1, P.batches.size()) * sizeof(DevBatch));
-  if (!P.batches.empty()) copy_h2d(ctx, P.d_batches.get(), P.batches.data(), P.batches.size() * sizeof(DevBatch));
+  if (!P.batches.empty()) upload(ctx, P.d_batches.get(), P.batches.data(), P.batches.size() * sizeof(DevBatch));
   P.d_lits = DevBuf(ctx, std::max<size_t>(1, P.lits.size()) * sizeof(DevLit));
   P.d_unit_cells = DevBuf(ctx, std::max<size_t>(1, P.unit_cells.size()) * 8);
-  if (!P.units.empty()) copy_h2d(ctx, P.d_units.get(), P.units.data(), P.units.size() * sizeof(DevUnit));
-  if (!P.lits.empty()) copy_h2d(ctx, P.d_lits.get(), P.lits.data(), P.lits.size() * sizeof(DevLit));
-  if (!P.unit_cells.empty()) copy_h2d(ctx, P.d_unit_cells.get(), P.unit_cells.data(), P.unit_cells.size() * 8);
+  if (!P.units.empty()) upload(ctx, P.d_units.get(), P.units.data(), P.units.size() * sizeof(DevUnit));
+  if (!P.lits.empty()) upload(ctx, P.d_lits.get(), P.lits.data(), P.lits.size() * sizeof(DevLit));
+  if (!P.unit_cells.empty()) upload(ctx, P.d_unit_cells.get(), P.unit_cells.data(), P.unit_cells.size() * 8);
   P.d_cell_counts = DevBuf(ctx, std::max<int64_t>(1, P.max_cells) * 8);
   P.d_cell_starts = DevBuf(ctx, std::max<int64_t>(1, P.max_cells) * 8);
   P.d_items = DevBuf(ctx, std::max<int64_t>(1, P.max_items) * sizeof(DevItem));
@@ -2579,6 +2681,7 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   P.ncopies = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(ctx->num_cus, P.max_blocks), (32ll << 20) / copy_bytes));
   P.d_copies = DevBuf(ctx, (size_t)(P.ncopies * copy_bytes));
   P.bit_words = ceil_div(binding.num_rows, 64);
+  end_uploads(ctx);
 }
 
 // Auxiliary streams for forked work: n - 1 streams and n events (event 0 forks, event v joins
@@ -2900,8 +3003,13 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
   }
   std::unique_ptr<StepPlan> P(new StepPlan);
   plan_step(ctx, binding, relations, specs, num_slots, o, *P, allow_stream && o.mode == QF_EVAL_BATCHED && d_bits == nullptr);
-  QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->pinned), (size_t)num_slots * 4 * sizeof(uint64_t),
-                             hipHostMallocDefault));
+  const size_t count_bytes = (size_t)num_slots * 4 * sizeof(uint64_t);
+  if (!key.empty()) {
+    QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->pinned), count_bytes, hipHostMallocDefault));
+    P->own_pinned = true;
+  } else {
+    P->pinned = stage_counts(ctx, count_bytes);
+  }
   if (!key.empty()) {
     // Capture the step's device work once; replays skip all host planning.
     // Temporaries released during capture stay reserved for the graph's lifetime.  Phase
@@ -3000,7 +3108,8 @@ static std::vector<GroupSpec> literal_groups(int binding_columns, const qf_table
     type_start[r + 1] = type_start[r] + relations[r].num_columns;
   }
   std::vector<GroupSpec> specs;
-  std::map<std::tuple<int, int, int>, int> index;
+  // Group of each (relation, join position, clause join variable).
+  std::vector<int> index((size_t)std::max(1, num_relations) * QF_MAX_ARITY * std::max(1, binding_columns), -1);
   for (int l = 0; l < num_literals; ++l) {
     const qf_literal& lit = literals[l];
     QF_REQUIRE(lit.relation >= 0 && lit.relation < num_relations, "literal relation out of range");
@@ -3012,26 +3121,28 @@ static std::vector<GroupSpec> literal_groups(int binding_columns, const qf_table
     QF_REQUIRE(jk >= 0, "literal has no bound variable");
     LitSpec ls;
     ls.slot = l;
+    ls.pcols.reserve(arity);
+    ls.bcols.reserve(arity);
     for (int i = 0; i < arity; ++i)
       if (lit.vars[i] >= 0) {
         ls.pcols.push_back(i);
         ls.bcols.push_back(lit.vars[i]);
       }
     normalise_key(ls);
-    const auto key = std::make_tuple(lit.relation, jk, lit.vars[jk]);
+    int& slot_of_key = index[((size_t)lit.relation * QF_MAX_ARITY + jk) * binding_columns + lit.vars[jk]];
     int gi;
-    if (o.mode == QF_EVAL_PLAIN_LITERAL || !index.count(key)) {
+    if (o.mode == QF_EVAL_PLAIN_LITERAL || slot_of_key < 0) {
       gi = (int)specs.size();
       GroupSpec g;
       g.relation = lit.relation;
       g.join_col = jk;
       g.join_var = lit.vars[jk];
       specs.push_back(g);
-      if (o.mode != QF_EVAL_PLAIN_LITERAL) index[key] = gi;
+      if (o.mode != QF_EVAL_PLAIN_LITERAL) slot_of_key = gi;
     } else {
-      gi = index[key];
+      gi = slot_of_key;
     }
-    specs[gi].lits.push_back(ls);
+    specs[gi].lits.push_back(std::move(ls));
   }
   return specs;
 }

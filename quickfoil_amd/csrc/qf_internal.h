@@ -111,6 +111,9 @@ struct Context {
   std::shared_ptr<void> graph_cache;
   // Autotuner choices per workload shape, owned by evaluate.hip.
   std::shared_ptr<void> tune_cache;
+  // Pinned host staging for a plan's uploads and the counts of uncaptured steps, owned by
+  // evaluate.hip.
+  std::shared_ptr<void> host_stage;
   // Private non-blocking stream for captured graphs (the caller's stream may be the legacy
   // default stream, which cannot be captured); ordered after the caller's stream by an event.
   hipStream_t graph_stream = nullptr;
