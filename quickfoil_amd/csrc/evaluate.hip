@@ -2125,6 +2125,8 @@ struct StepPlan {
   std::vector<std::vector<DevBuf>> tparts;  // per join variable: staged columns + row id
   std::vector<DevBuf> toffs;
   std::vector<std::vector<int>> pgroups;    // join variables partitioned together (indices)
+  std::map<std::pair<int, int>, int> unit_R;  // radix bits per (relation, join column)
+  bool partitioned = false;                 // the binding partition is already enqueued
   std::vector<DevBuf> tscratch;             // per group: its partition's temporaries
   std::vector<std::shared_ptr<RelPart>> keep_rel;
   std::vector<std::shared_ptr<HeavyTables>> keep_heavy;
@@ -2170,15 +2172,15 @@ struct StepPlan {
 // literal-group (plain group), or one literal each (plain literal).
 // stream: the batched form without covered bitvectors, whose units k_stream counts from their
 // relations' distinct key tuples in the cells whose binding rows fit its work items.
-static void plan_step(Context* ctx, const qf_table& binding, const qf_table* relations,
-                      const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P,
-                      bool stream) {
-  const bool cache = o.reuse_partitions != 0;
-  begin_uploads(ctx);
+// The plan in two parts: plan_binding fixes what the binding partition needs (radix bits,
+// staged columns, join variables, the partitions' buffers), so that an uncaptured step can start
+// the partition while plan_units builds the rest.
+static void plan_binding(Context* ctx, const qf_table& binding, const qf_table* relations,
+                         const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P) {
   P.num_slots = num_slots;
   P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
   P.tile = (P.tile + 63) & ~63;  // whole 64-row blocks
-  std::map<std::pair<int, int>, int> unit_R;
+  std::map<std::pair<int, int>, int>& unit_R = P.unit_R;
   for (const GroupSpec& g : specs) {
     const int R = o.radix_bits > 0 ? std::min(16, o.radix_bits)
                                    : auto_radix_bits(relations[g.relation].num_rows,
@@ -2226,6 +2228,14 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
     const int part_cols = shared_pass ? (int)P.bcols.size() + 1 : (int)std::min<size_t>(P.bcols.size(), 7) + 1;
     P.tscratch.emplace_back(ctx, std::max<size_t>(256, partition_scratch_bytes(binding.num_rows, part_cols, P.R_T, nk)));
   }
+}
+
+static void plan_units(Context* ctx, const qf_table& binding, const qf_table* relations,
+                       const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P,
+                       bool stream) {
+  const bool cache = o.reuse_partitions != 0;
+  begin_uploads(ctx);
+  const std::map<std::pair<int, int>, int>& unit_R = P.unit_R;
   auto var_index = [&](int v) { return (int)(std::find(P.join_vars.begin(), P.join_vars.end(), v) - P.join_vars.begin()); };
   auto tidx = [&](int c) { return (int)(std::find(P.bcols.begin(), P.bcols.end(), c) - P.bcols.begin()); };
 
@@ -2684,6 +2694,13 @@ static void plan_step(Context* ctx, const qf_table& binding, const qf_table* rel
   end_uploads(ctx);
 }
 
+static void plan_step(Context* ctx, const qf_table& binding, const qf_table* relations,
+                      const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P,
+                      bool stream) {
+  plan_binding(ctx, binding, relations, specs, num_slots, o, P);
+  plan_units(ctx, binding, relations, specs, num_slots, o, P, stream);
+}
+
 // Auxiliary streams for forked work: n - 1 streams and n events (event 0 forks, event v joins
 // stream v - 1 back).  Default priority, non-blocking.
 static void ensure_aux_streams(Context* ctx, int n) {
@@ -2699,6 +2716,56 @@ static void ensure_aux_streams(Context* ctx, int n) {
   }
 }
 
+// The binding partitions of the plan's join variables (group by group, forked streams joined
+// back to the context stream).
+static void enqueue_partition(Context* ctx, const qf_table& binding, StepPlan& P) {
+  PhaseScope ps(ctx, "partition_binding");
+  const int ng = (int)P.pgroups.size();
+  ensure_aux_streams(ctx, ng);
+  hipStream_t main = ctx->stream;
+  if (ng > 1) QF_CHECK_HIP(hipEventRecord(ctx->aux_events[0], main));
+  for (int g = 0; g < ng; ++g) {
+    // Group g on aux stream g - 1 (the first on the context stream), after all work already
+    // queued on the context stream.
+    hipStream_t st = g == 0 ? main : ctx->aux_streams[g - 1];
+    if (g > 0) QF_CHECK_HIP(hipStreamWaitEvent(st, ctx->aux_events[0], 0));
+    Scratch scratch;
+    scratch.base = P.tscratch[g].as<char>();
+    scratch.capacity = P.tscratch[g].bytes();
+    ctx->stream = st;
+    try {
+      const std::vector<int>& grp = P.pgroups[g];
+      if (grp.size() > 1) {
+        std::vector<const int32_t*> in;
+        for (int c : P.bcols) in.push_back(binding.columns[c]);
+        std::vector<int> keys;
+        std::vector<int32_t*> outs;
+        std::vector<int64_t*> offs;
+        for (int v : grp) {
+          keys.push_back((int)(std::find(P.bcols.begin(), P.bcols.end(), P.join_vars[v]) - P.bcols.begin()));
+          for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
+          offs.push_back(P.toffs[v].as<int64_t>());
+        }
+        stable_partition_multi(ctx, in.data(), (int)in.size(), keys.data(), (int)keys.size(), true,
+                               binding.num_rows, 1, P.R_T, outs.data(), offs.data(), &scratch);
+      } else {
+        const int v = grp[0];
+        std::vector<int32_t*> outs;
+        for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
+        partition_binding(ctx, binding, P.join_vars[v], P.bcols, P.R_T, outs, P.toffs[v].as<int64_t>(), &scratch);
+      }
+    } catch (...) {
+      ctx->stream = main;
+      throw;
+    }
+    ctx->stream = main;
+    if (g > 0) {
+      QF_CHECK_HIP(hipEventRecord(ctx->aux_events[g], st));
+      QF_CHECK_HIP(hipStreamWaitEvent(main, ctx->aux_events[g], 0));
+    }
+  }
+}
+
 // Device work of one step on the context stream, the binding partitions forked to auxiliary
 // streams and joined back (capturable: kernels, device copies and event waits).
 static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_positive, StepPlan& P, uint64_t* d_bits,
@@ -2706,53 +2773,7 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
   fill_u64(ctx, P.d_copies.as<uint64_t>(), 0, (int64_t)P.ncopies * P.num_slots * 4);
   if (d_bits && P.num_slots > 0) fill_u64(ctx, d_bits, 0, (int64_t)P.num_slots * P.bit_words);
   if (!P.units.empty() && binding.num_rows > 0) {
-    {
-      PhaseScope ps(ctx, "partition_binding");
-      const int ng = (int)P.pgroups.size();
-      ensure_aux_streams(ctx, ng);
-      hipStream_t main = ctx->stream;
-      if (ng > 1) QF_CHECK_HIP(hipEventRecord(ctx->aux_events[0], main));
-      for (int g = 0; g < ng; ++g) {
-        // Group g on aux stream g - 1 (the first on the context stream), after all work already
-        // queued on the context stream.
-        hipStream_t st = g == 0 ? main : ctx->aux_streams[g - 1];
-        if (g > 0) QF_CHECK_HIP(hipStreamWaitEvent(st, ctx->aux_events[0], 0));
-        Scratch scratch;
-        scratch.base = P.tscratch[g].as<char>();
-        scratch.capacity = P.tscratch[g].bytes();
-        ctx->stream = st;
-        try {
-          const std::vector<int>& grp = P.pgroups[g];
-          if (grp.size() > 1) {
-            std::vector<const int32_t*> in;
-            for (int c : P.bcols) in.push_back(binding.columns[c]);
-            std::vector<int> keys;
-            std::vector<int32_t*> outs;
-            std::vector<int64_t*> offs;
-            for (int v : grp) {
-              keys.push_back((int)(std::find(P.bcols.begin(), P.bcols.end(), P.join_vars[v]) - P.bcols.begin()));
-              for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
-              offs.push_back(P.toffs[v].as<int64_t>());
-            }
-            stable_partition_multi(ctx, in.data(), (int)in.size(), keys.data(), (int)keys.size(), true,
-                                   binding.num_rows, 1, P.R_T, outs.data(), offs.data(), &scratch);
-          } else {
-            const int v = grp[0];
-            std::vector<int32_t*> outs;
-            for (auto& b : P.tparts[v]) outs.push_back(b.as<int32_t>());
-            partition_binding(ctx, binding, P.join_vars[v], P.bcols, P.R_T, outs, P.toffs[v].as<int64_t>(), &scratch);
-          }
-        } catch (...) {
-          ctx->stream = main;
-          throw;
-        }
-        ctx->stream = main;
-        if (g > 0) {
-          QF_CHECK_HIP(hipEventRecord(ctx->aux_events[g], st));
-          QF_CHECK_HIP(hipStreamWaitEvent(main, ctx->aux_events[g], 0));
-        }
-      }
-    }
+    if (!P.partitioned) enqueue_partition(ctx, binding, P);
     // Streamed units: the cells they take, then launch by launch.
     const int64_t cells_T = (int64_t)1 << P.R_T;
     if (!P.slaunches.empty()) {
@@ -3002,7 +3023,18 @@ static void run_groups_impl(Context* ctx, const qf_table& binding, int64_t num_p
     }
   }
   std::unique_ptr<StepPlan> P(new StepPlan);
-  plan_step(ctx, binding, relations, specs, num_slots, o, *P, allow_stream && o.mode == QF_EVAL_BATCHED && d_bits == nullptr);
+  const bool streamable = allow_stream && o.mode == QF_EVAL_BATCHED && d_bits == nullptr;
+  if (key.empty()) {
+    // Uncaptured: the binding partition runs while the rest of the plan is built.
+    plan_binding(ctx, binding, relations, specs, num_slots, o, *P);
+    if (binding.num_rows > 0 && !specs.empty()) {
+      enqueue_partition(ctx, binding, *P);
+      P->partitioned = true;
+    }
+    plan_units(ctx, binding, relations, specs, num_slots, o, *P, streamable);
+  } else {
+    plan_step(ctx, binding, relations, specs, num_slots, o, *P, streamable);
+  }
   const size_t count_bytes = (size_t)num_slots * 4 * sizeof(uint64_t);
   if (!key.empty()) {
     QF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&P->pinned), count_bytes, hipHostMallocDefault));
