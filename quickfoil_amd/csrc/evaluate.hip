@@ -2040,13 +2040,20 @@ static void partition_binding(Context* ctx, const qf_table& b, int var, const st
 struct HostStage {
   char* base = nullptr;
   size_t cap = 0, used = 0, want = 0;
-  hipEvent_t done = nullptr;
+  hipEvent_t done = nullptr, before = nullptr;
+  hipStream_t stream = nullptr;  // the uploads' stream (default priority)
+  uint64_t releases = 0;         // pool releases the upload stream is ordered after
   bool pending = false;
   uint64_t* counts = nullptr;
   size_t counts_cap = 0;
   ~HostStage() {
     if (pending && done) (void)hipEventSynchronize(done);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
     if (done) (void)hipEventDestroy(done);
+    if (before) (void)hipEventDestroy(before);
     if (base) (void)hipHostFree(base);
     if (counts) (void)hipHostFree(counts);
   }
@@ -2057,16 +2064,25 @@ static HostStage& host_stage(Context* ctx) {
   return *std::static_pointer_cast<HostStage>(ctx->host_stage);
 }
 
+// The uploads run on a stream of their own, after the work queued on the context stream before
+// the plan began (begin_uploads) and before the work queued after it ended (end_uploads) or
+// after uploads_before, so that they overlap a binding partition queued in between.
+
 // Starts a plan's uploads: the previous plan's copies are done with the region.
 static void begin_uploads(Context* ctx) {
   HostStage& H = host_stage(ctx);
+  if (!H.stream) QF_CHECK_HIP(hipStreamCreateWithFlags(&H.stream, hipStreamNonBlocking));
+  if (!H.done) QF_CHECK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
+  if (!H.before) QF_CHECK_HIP(hipEventCreateWithFlags(&H.before, hipEventDisableTiming));
   if (H.pending) {
     QF_CHECK_HIP(hipEventSynchronize(H.done));
     H.pending = false;
   } else if (H.used > 0) {
-    sync(ctx);  // a plan that stopped early left copies without the event
+    QF_CHECK_HIP(hipStreamSynchronize(H.stream));  // a plan that stopped early left copies without the event
   }
-  if (!H.done) QF_CHECK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
+  QF_CHECK_HIP(hipEventRecord(H.before, ctx->stream));
+  QF_CHECK_HIP(hipStreamWaitEvent(H.stream, H.before, 0));
+  H.releases = ctx->pool->releases();
   if (H.want > H.cap) {
     if (H.base) QF_CHECK_HIP(hipHostFree(H.base));
     H.base = nullptr;
@@ -2079,28 +2095,43 @@ static void begin_uploads(Context* ctx) {
   H.want = 0;
 }
 
-// Host-to-device copy of a plan upload on the context stream (see HostStage).
+// Host-to-device copy of a plan upload on the upload stream (see HostStage).  The destination
+// may be a pool block released since the upload stream last waited for the context stream (the
+// pool hands blocks to later work of the same stream without a wait): then it waits again.
 static void upload(Context* ctx, void* dst, const void* src, size_t bytes) {
   if (!bytes) return;
   HostStage& H = host_stage(ctx);
+  if (ctx->pool->releases() != H.releases) {
+    QF_CHECK_HIP(hipEventRecord(H.before, ctx->stream));
+    QF_CHECK_HIP(hipStreamWaitEvent(H.stream, H.before, 0));
+    H.releases = ctx->pool->releases();
+  }
   const size_t at = (H.used + 255) & ~(size_t)255;
   if (H.base && at + bytes <= H.cap) {
     std::memcpy(H.base + at, src, bytes);
-    QF_CHECK_HIP(hipMemcpyAsync(dst, H.base + at, bytes, hipMemcpyHostToDevice, ctx->stream));
+    QF_CHECK_HIP(hipMemcpyAsync(dst, H.base + at, bytes, hipMemcpyHostToDevice, H.stream));
     H.used = at + bytes;
   } else {
     H.want = std::max(H.want, at + bytes);
-    copy_h2d(ctx, dst, src, bytes);
+    QF_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, H.stream));
+    QF_CHECK_HIP(hipStreamSynchronize(H.stream));
   }
 }
 
-// Ends a plan's uploads: the event marks when the region may be reused.
+// The context stream's later work waits for the uploads so far.
+static void uploads_before(Context* ctx) {
+  HostStage& H = host_stage(ctx);
+  QF_CHECK_HIP(hipEventRecord(H.done, H.stream));
+  QF_CHECK_HIP(hipStreamWaitEvent(ctx->stream, H.done, 0));
+}
+
+// Ends a plan's uploads: the context stream's later work waits for them; the event also marks
+// when the region may be reused.
 static void end_uploads(Context* ctx) {
   HostStage& H = host_stage(ctx);
-  if (H.used > 0) {
-    QF_CHECK_HIP(hipEventRecord(H.done, ctx->stream));
-    H.pending = true;
-  }
+  QF_CHECK_HIP(hipEventRecord(H.done, H.stream));
+  QF_CHECK_HIP(hipStreamWaitEvent(ctx->stream, H.done, 0));
+  H.pending = true;
 }
 
 // Pinned counts for a step run without a graph (the call synchronises before it returns).
@@ -2177,6 +2208,7 @@ struct StepPlan {
 // the partition while plan_units builds the rest.
 static void plan_binding(Context* ctx, const qf_table& binding, const qf_table* relations,
                          const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P) {
+  begin_uploads(ctx);
   P.num_slots = num_slots;
   P.tile = o.tile_rows > 0 ? std::min(4096, std::max(64, o.tile_rows)) : 1024;
   P.tile = (P.tile + 63) & ~63;  // whole 64-row blocks
@@ -2234,7 +2266,6 @@ static void plan_units(Context* ctx, const qf_table& binding, const qf_table* re
                        const std::vector<GroupSpec>& specs, int num_slots, const qf_eval_options& o, StepPlan& P,
                        bool stream) {
   const bool cache = o.reuse_partitions != 0;
-  begin_uploads(ctx);
   const std::map<std::pair<int, int>, int>& unit_R = P.unit_R;
   auto var_index = [&](int v) { return (int)(std::find(P.join_vars.begin(), P.join_vars.end(), v) - P.join_vars.begin()); };
   auto tidx = [&](int c) { return (int)(std::find(P.bcols.begin(), P.bcols.end(), c) - P.bcols.begin()); };
@@ -2634,6 +2665,7 @@ static void plan_units(Context* ctx, const qf_table& binding, const qf_table* re
       // The cells' list ranges depend on the relations' key-tuple lists only: made once here.
       SL.d_parts = DevBuf(ctx, cells * 4);
       SL.d_cellmat = DevBuf(ctx, (size_t)cells * std::max<size_t>(1, pk.lists.size()) * sizeof(int2));
+      uploads_before(ctx);
       hipLaunchKernelGGL(k_ts_cells, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, ctx->stream,
                          SL.d_launch.as<TsLaunch>(), SL.d_parts.as<int32_t>(), SL.d_cellmat.as<int2>());
       QF_CHECK_HIP(hipGetLastError());

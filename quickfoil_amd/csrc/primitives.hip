@@ -56,6 +56,7 @@ void Pool::put(void* p) {
   if (it == live_.end()) return;
   free_.emplace(it->second, p);
   live_.erase(it);
+  ++releases_;
 }
 
 void Pool::release_all() {

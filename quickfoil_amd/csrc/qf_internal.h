@@ -47,10 +47,14 @@ class Pool {
   void* get(size_t bytes);
   void put(void* p);
   void release_all();
+  // Blocks released so far: work on another stream that writes a block taken after a release
+  // must first wait for the work queued before it.
+  uint64_t releases() const { return releases_; }
 
  private:
   static size_t size_class(size_t bytes);
   int device_;
+  uint64_t releases_ = 0;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> live_;
 };
