@@ -239,8 +239,10 @@ def main():
     ctx = quickfoil_amd.Context(0)
     ctx.enable_timing(True)
     L = len(w["literals"])
-    # The literal list in the C layout, made once (a host learner hands the library such an array).
+    # The literal list and the relations' table descriptors in the C layout, made once (a host
+    # learner hands the library such arrays).
     lits = quickfoil_amd.LiteralArray(w["literals"])
+    rels = quickfoil_amd.TableSet(w["relations"])
 
     # The batched form takes its facts per partition and tile from the autotuner (it times a grid
     # of both on the first call for this workload shape); the plain form reuses the choice.
@@ -248,7 +250,7 @@ def main():
 
     def step(mode):
         auto = mode == "batched" and not args.no_autotune and not args.radix_bits and not args.tile_rows
-        counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], lits, mode=mode,
+        counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], rels, lits, mode=mode,
                                           reuse_partitions=True, radix_bits=tune["radix_bits"],
                                           tile_rows=0 if auto else tune["tile_rows"],
                                           partition_rows=0 if auto else tune["partition_rows"],
@@ -257,7 +259,7 @@ def main():
 
     # Cold step: nothing cached, default facts per partition and tile (partitions the relations).
     tc = time.perf_counter()
-    ref_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], lits,
+    ref_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], rels, lits,
                                           mode="batched", reuse_partitions=True, radix_bits=args.radix_bits,
                                           tile_rows=args.tile_rows)
     torch.cuda.synchronize()
@@ -298,7 +300,7 @@ def main():
     # cross-checks them.
     ctx.reset_timing()
     for i in range(args.steps):
-        eager_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], w["relations"], lits,
+        eager_counts, _ = ctx.evaluate_literals(w["binding"], w["num_positive"], rels, lits,
                                                 mode="batched", reuse_partitions=True, **eager_kw)
     torch.cuda.synchronize()
     assert np.array_equal(eager_counts, ref_counts), "eager and graph counts differ"
@@ -313,7 +315,7 @@ def main():
             fresh = [c.clone() for c in w["binding"]]
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            fresh_counts, _ = ctx.evaluate_literals(fresh, w["num_positive"], w["relations"], lits, mode="batched",
+            fresh_counts, _ = ctx.evaluate_literals(fresh, w["num_positive"], rels, lits, mode="batched",
                                                     reuse_partitions=True, **eager_kw)
             times.append(time.perf_counter() - t0)
             assert np.array_equal(fresh_counts, ref_counts), "counts on a new binding table differ"

@@ -187,6 +187,19 @@ class _TableHolder:
         self.t = QfTable(ctypes.cast(self.arr, ctypes.POINTER(ctypes.c_void_p)), len(cols), num_rows)
 
 
+class TableSet:
+    """Device tables (each a list of contiguous int32 GPU column tensors) converted once to the C
+    layout, an array of qf_table, for callers that pass the same tables on every call (a learner's
+    relations).  The tensors stay referenced; their contents may change, their storage may not."""
+
+    def __init__(self, tables):
+        self.holders = [_TableHolder(t) for t in tables]
+        self.array = (QfTable * max(1, len(self.holders)))(*[h.t for h in self.holders])
+
+    def __len__(self):
+        return len(self.holders)
+
+
 class Context:
     """A qf_context bound to torch's current stream on one GPU."""
 
@@ -301,12 +314,16 @@ class Context:
         """Returns (counts[L,4] uint64 ndarray, bits tensor or None).
 
         literals: list of (relation index, [var per position]) with -1 for new variables, or a
-        LiteralArray made from one.  Counts are {covered_pos, covered_neg, binding_pos, binding_neg}.
+        LiteralArray made from one; relations: a list of column lists, or a TableSet made from one.
+        Counts are {covered_pos, covered_neg, binding_pos, binding_neg}.
         """
         t = self.torch
         b = _TableHolder(binding_columns)
-        rels = [_TableHolder(r) for r in relations]
-        rel_arr = (QfTable * max(1, len(rels)))(*[r.t for r in rels])
+        if isinstance(relations, TableSet):
+            rels, rel_arr = relations.holders, relations.array
+        else:
+            rels = [_TableHolder(r) for r in relations]
+            rel_arr = (QfTable * max(1, len(rels)))(*[r.t for r in rels])
         if not isinstance(literals, LiteralArray):
             literals = LiteralArray(literals)
         L = len(literals)

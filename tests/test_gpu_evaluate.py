@@ -243,6 +243,25 @@ def test_graph_replay_follows_changed_calls(qf_ctx):
     qf_ctx.trim()
 
 
+def test_table_set_matches_column_lists(qf_ctx):
+    """Relations handed over once as a TableSet count exactly as the same column lists, eagerly
+    and through graph replays."""
+    import quickfoil_amd
+    binding, npos, rels, lits = random_problem(47, n=9000, rel_shapes=((2, 6000), (3, 4000), (1, 300)), domain=120,
+                                               zipf=True, max_literals=40)
+    want, _ = oracle.Problem(binding, npos, rels, lits).grouped()
+    dev_b = [_t(c) for c in binding]
+    dev_r = [[_t(c) for c in r] for r in rels]
+    ts = quickfoil_amd.TableSet(dev_r)
+    assert len(ts) == len(rels)
+    got, _ = qf_ctx.evaluate_literals(dev_b, npos, ts, lits)
+    np.testing.assert_array_equal(got, want)
+    for _ in range(2):
+        got, _ = qf_ctx.evaluate_literals(dev_b, npos, ts, lits, use_graph=True, reuse_partitions=True)
+        np.testing.assert_array_equal(got, want)
+    qf_ctx.trim()
+
+
 def test_autotuned_step_matches_golden(qf_ctx):
     """autotune=1 times its (facts per partition, tile) grid on the first call for a workload shape
     and runs the fastest; the counts of the 1/256 refinement step equal the committed golden ones,
