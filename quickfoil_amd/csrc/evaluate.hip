@@ -604,11 +604,13 @@ __device__ inline void ts_pair_probes(const TsArgs& a, cptr<TsList> lists, const
   // from its current one to the group's end and stops at the first empty slot.  The common case
   // (no fingerprint hit) is branch-free: bit masks of the group's empty slots and fingerprint
   // hits; only a hit goes to the key check.
+  // Every lane reads all four groups (a slot is always inside the table), so that the reads need
+  // no exec-mask changes; a finished probe's group is read and ignored.
   const uint4* groups = reinterpret_cast<const uint4*>(pword);
   while (live[0] || live[1] || live[2] || live[3]) {
     uint4 g[TS_TJ];
 #pragma unroll
-    for (int j = 0; j < TS_TJ; ++j) g[j] = live[j] ? groups[slot[j] >> 2] : make_uint4(0u, 0u, 0u, 0u);
+    for (int j = 0; j < TS_TJ; ++j) g[j] = groups[slot[j] >> 2];
 #pragma unroll
     for (int j = 0; j < TS_TJ; ++j) {
       const uint32_t wds[4] = {g[j].x, g[j].y, g[j].z, g[j].w};
