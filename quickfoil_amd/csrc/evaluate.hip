@@ -914,7 +914,7 @@ __global__ __launch_bounds__(TS_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
           while (look[0] || look[1] || look[2] || look[3]) {
             unsigned long long wd[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) wd[u] = look[u] ? vtab[slot[u]] : 0ull;
+            for (int u = 0; u < 4; ++u) wd[u] = vtab[slot[u]];  // always inside the table; read unmasked
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               if (!look[u]) continue;
