@@ -2937,6 +2937,8 @@ static void call_signature(std::string& out, const qf_table& binding, int64_t nu
   put(binding.columns, (size_t)binding.num_columns * sizeof(void*));
   int64_t ntypes = 0;
   for (int r = 0; r < num_relations; ++r) {
+    QF_REQUIRE(relations[r].num_columns >= 1 && relations[r].num_columns <= QF_MAX_ARITY && relations[r].columns,
+               "relation has no columns or more than QF_MAX_ARITY");
     const int64_t shape[] = {relations[r].num_rows, relations[r].num_columns};
     put(shape, sizeof(shape));
     put(relations[r].columns, (size_t)relations[r].num_columns * sizeof(void*));
