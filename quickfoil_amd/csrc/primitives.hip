@@ -330,12 +330,11 @@ __device__ inline void load_tile(const PassArgs& a, int ki, int64_t tile, int la
 }
 
 // Persistent workgroups, software pipelined: a tile's rows are loaded into registers while the
-// previous tile is written out of LDS.  SHARED (a first pass over one input for every key): a
-// workgroup walks the input's tiles and scatters each tile once per key; otherwise it walks the
-// tiles of all keys' inputs, key after key.
+// previous tile is written out of LDS.  A workgroup walks the tiles of all keys' inputs, key after
+// key (a first pass over one input for several keys is k_part_scatter_keys).
 // NC = a.ncols.  4096-row tiles: at 7-bit digits a bin's run in a tile averages 32 rows (128 B per
 // column), which measured 10-16 % faster than 2048-row tiles at the same number of waves per CU.
-template <int NC, bool SHARED>
+template <int NC>
 __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_part_scatter(
     PassArgs a, const int64_t* bases) {
   extern __shared__ int32_t stage[];                // ncols x PT_TILE rows in bin order, then digits
@@ -353,15 +352,14 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   const int wsh = 16 * (w & 1);
   // Walk position g -> (key of the input, tile of that input).
   const int64_t T = a.num_tiles;
-  TileWalk walk(SHARED ? T : T * a.nkeys);
+  TileWalk walk(T * a.nkeys);
   int32_t v[PT_IPT][NC];
-  if (walk.next < walk.end)
-    load_tile<NC>(a, SHARED ? 0 : (int)(walk.next / T), SHARED ? walk.next : walk.next % T, lane, w, v);
+  if (walk.next < walk.end) load_tile<NC>(a, (int)(walk.next / T), walk.next % T, lane, w, v);
 
   for (int64_t g = walk.next; g < walk.end; g += walk.step) {
-    const int kin = SHARED ? 0 : (int)(g / T);
-    const int64_t tile = SHARED ? g : g % T;
-    const int k_end = SHARED ? a.nkeys : kin + 1;
+    const int kin = (int)(g / T);
+    const int64_t tile = g % T;
+    const int k_end = kin + 1;
     // Each wave owns PT_IPT consecutive 64-row groups of the tile: (wave, group, lane) is row order.
     const int64_t tile_base = tile * PT_TILE;
     const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
@@ -446,7 +444,7 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
       // After the last key's staging the next tile's loads fly while this one is written out.
       if (k == k_end - 1 && g + walk.step < walk.end) {
         const int64_t gn = g + walk.step;
-        load_tile<NC>(a, SHARED ? 0 : (int)(gn / T), SHARED ? gn : gn % T, lane, w, v);
+        load_tile<NC>(a, (int)(gn / T), gn % T, lane, w, v);
       }
 
       // Consecutive threads store consecutive rows of a bin.
@@ -457,6 +455,141 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
         for (int c = 0; c < NC; ++c) out[c][dest * a.out_stride] = stage[c * PT_TILE + j];
       }
+    }
+  }
+}
+
+// The first pass over one input for every key (the binding table by each clause join variable):
+// the tile's digit counts of all keys are taken in one phase and scanned in one phase (a thread
+// per key and bin), then the tile is staged and written out once per key: two barriers per key
+// after the first instead of five.
+template <int NC>
+__global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_part_scatter_keys(
+    PassArgs a, const int64_t* bases) {
+  extern __shared__ int32_t stage[];  // ncols x PT_TILE rows in bin order, then digits
+  // Per key: per-wave counts, then per-wave running starts, 16 bits each (wave w uses half
+  // (w & 1) of row (w >> 1)).
+  __shared__ uint32_t wrun[PT_MAXKEYS][PT_WAVES / 2][PT_MAXBINS];
+  __shared__ int64_t gbase[PT_MAXKEYS][PT_MAXBINS];  // output row of bin b's first tile row, minus its tile start
+  __shared__ uint32_t wtot[PT_WAVES];
+  static_assert(PT_MAXKEYS * PT_MAXBINS == PT_THREADS, "one scan thread per key and bin");
+  uint8_t* dig8 = reinterpret_cast<uint8_t*>(stage + a.ncols * PT_TILE);
+
+  const int nb = 1 << a.bits;
+  const int nk = a.nkeys;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int wsh = 16 * (w & 1);
+  const int sk = threadIdx.x / PT_MAXBINS, sb = threadIdx.x % PT_MAXBINS;  // the thread's key and bin in the scan
+  const int64_t T = a.num_tiles;
+  TileWalk walk(T);
+  for (int b = threadIdx.x; b < nk * (PT_WAVES / 2) * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0][0])[b] = 0;
+  int32_t v[PT_IPT][NC];
+  if (walk.next < walk.end) load_tile<NC>(a, 0, walk.next, lane, w, v);
+
+  for (int64_t tile = walk.next; tile < walk.end; tile += walk.step) {
+    const int64_t tile_base = tile * PT_TILE;
+    const int64_t wave_base = tile_base + (int64_t)w * 64 * PT_IPT;
+    // Every key's digit of the lane's rows, a byte each.
+    uint32_t d4[PT_IPT];
+#pragma unroll
+    for (int i = 0; i < PT_IPT; ++i) {
+      d4[i] = 0;
+#pragma unroll
+      for (int k = 0; k < PT_MAXKEYS; ++k) {
+        if (k >= nk) break;
+        const int kc = a.key_col[k];
+        int32_t key = v[i][0];
+#pragma unroll
+        for (int c = 1; c < NC; ++c)
+          if (c == kc) key = v[i][c];
+        d4[i] |= digit_of(key, a.mixed, a.shift, a.bits) << (8 * k);
+      }
+    }
+    __syncthreads();  // counts cleared; the previous tile's write-out is done with LDS
+#pragma unroll
+    for (int i = 0; i < PT_IPT; ++i)
+      if (wave_base + (int64_t)i * 64 + lane < a.n)
+        for (int k = 0; k < nk; ++k) atomicAdd(&wrun[k][w >> 1][(d4[i] >> (8 * k)) & 0xffu], 1u << wsh);
+    __syncthreads();
+
+    // Bin totals of every key, their exclusive scans (64 bins per wave), per-wave starts.
+    uint32_t tot = 0;
+    if (sk < nk && sb < nb)
+      for (int ww = 0; ww < PT_WAVES / 2; ++ww) {
+        const uint32_t pair = wrun[sk][ww][sb];
+        tot += (pair & 0xffffu) + (pair >> 16);
+      }
+    uint32_t incl = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    if (sk < nk && sb < nb) {
+      uint32_t excl = incl - tot;
+      for (int ww = sk * (PT_MAXBINS / 64); ww < w; ++ww) excl += wtot[ww];
+      uint32_t at = excl;
+      for (int ww = 0; ww < PT_WAVES / 2; ++ww) {
+        const uint32_t pair = wrun[sk][ww][sb];
+        const uint32_t lo = at, hi = at + (pair & 0xffffu);
+        wrun[sk][ww][sb] = lo | (hi << 16);
+        at = hi + (pair >> 16);
+      }
+      // The scan over all keys' counts puts key k's rows after the k * n rows of the keys before.
+      gbase[sk][sb] = bases[((int64_t)sk * nb + sb) * T + tile] - (int64_t)sk * a.n - (int64_t)excl;
+    }
+    __syncthreads();
+
+    const int tile_rows = (int)min((int64_t)PT_TILE, a.n - tile_base);
+    for (int k = 0; k < nk; ++k) {
+      // Stable ranks within the wave (peers: lanes of the group with the same digit, one ballot per
+      // digit bit), rows staged in LDS in bin order.
+#pragma unroll
+      for (int i = 0; i < PT_IPT; ++i) {
+        const int64_t row = wave_base + (int64_t)i * 64 + lane;
+        const bool valid = row < a.n;
+        const uint64_t vmask = __ballot(valid);
+        if (vmask == 0) break;
+        const uint32_t d = (d4[i] >> (8 * k)) & 0xffu;
+        uint64_t peers = vmask;
+        for (int kb = 0; kb < a.bits; ++kb) {
+          const bool bit = (d >> kb) & 1u;
+          const uint64_t bk = __ballot(bit);
+          peers &= bit ? bk : ~bk;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        uint32_t pos = 0;
+        if (valid) pos = ((wrun[k][w >> 1][d] >> wsh) & 0xffffu) + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) stage[c * PT_TILE + pos] = v[i][c];
+          dig8[pos] = (uint8_t)d;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) atomicAdd(&wrun[k][w >> 1][d], (uint32_t)__popcll(peers) << wsh);
+        __builtin_amdgcn_wave_barrier();
+      }
+      __syncthreads();
+
+      if (k == nk - 1) {
+        // No rank reads the counts any more: clear them for the next tile, whose loads fly while
+        // this one is written out.
+        for (int b = threadIdx.x; b < nk * (PT_WAVES / 2) * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0][0])[b] = 0;
+        if (tile + walk.step < walk.end) load_tile<NC>(a, 0, tile + walk.step, lane, w, v);
+      }
+
+      // Consecutive threads store consecutive rows of a bin.
+      int32_t* const* out = a.out[k];
+      for (int j = threadIdx.x; j < tile_rows; j += PT_THREADS) {
+        const int64_t dest = gbase[k][dig8[j]] + j;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) out[c][dest * a.out_stride] = stage[c * PT_TILE + j];
+      }
+      if (k < nk - 1) __syncthreads();  // the next key's staging overwrites the tile
     }
   }
 }
@@ -498,25 +631,53 @@ __global__ void k_part_offsets(OffsetArgs o, int stride, int64_t n, int mixed, i
   }
 }
 
-template <int NC, bool SHARED>
-static void launch_scatter_nc(Context* ctx, const PassArgs& a, const int64_t* bases) {
+template <int NC>
+static void launch_scatter_keys(Context* ctx, const PassArgs& a, const int64_t* bases) {
+  const size_t lds = scatter_lds_bytes(NC);
+  static bool lds_attr[64] = {};
+  int dev = 0;
+  QF_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64 || !lds_attr[dev]) {
+    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter_keys<NC>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (dev >= 0 && dev < 64) lds_attr[dev] = true;
+  }
+  int per_cu = 0;
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter_keys<NC>, PT_THREADS, lds));
+  QF_REQUIRE(per_cu >= 1, "the multi-key scatter does not fit a CU");
+  int64_t grid = (int64_t)per_cu * ctx->num_cus;
+  if (grid >= 8) grid &= ~int64_t(7);
+  grid = std::min<int64_t>(grid, a.num_tiles);
+  hipLaunchKernelGGL((k_part_scatter_keys<NC>), dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
+}
+
+template <int NC>
+static void launch_scatter_one(Context* ctx, const PassArgs& a, const int64_t* bases) {
   const size_t lds = scatter_lds_bytes(NC);
   // Once per device (outside any graph capture in practice: the first pass runs eagerly).
   static bool lds_attr[64] = {};
   int dev = 0;
   QF_CHECK_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64 || !lds_attr[dev]) {
-    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter<NC, SHARED>),
+    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter<NC>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (dev >= 0 && dev < 64) lds_attr[dev] = true;
   }
   // Persistent grid: as many workgroups as fit the GPU at once (a multiple of 8), at most one per tile.
   int per_cu = 0;
-  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter<NC, SHARED>, PT_THREADS, lds));
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter<NC>, PT_THREADS, lds));
   int64_t grid = (int64_t)std::max(1, per_cu) * ctx->num_cus;
   if (grid >= 8) grid &= ~int64_t(7);
-  grid = std::min<int64_t>(grid, SHARED ? a.num_tiles : a.num_tiles * a.nkeys);
-  hipLaunchKernelGGL((k_part_scatter<NC, SHARED>), dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
+  grid = std::min<int64_t>(grid, a.num_tiles * a.nkeys);
+  hipLaunchKernelGGL((k_part_scatter<NC>), dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
+}
+
+template <int NC, bool SHARED>
+static void launch_scatter_nc(Context* ctx, const PassArgs& a, const int64_t* bases) {
+  if constexpr (SHARED)
+    launch_scatter_keys<NC>(ctx, a, bases);
+  else
+    launch_scatter_one<NC>(ctx, a, bases);
 }
 
 template <bool SHARED>
