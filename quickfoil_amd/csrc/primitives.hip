@@ -713,7 +713,9 @@ static void run_pass(Context* ctx, PassArgs a, bool shared, Scratch* scratch) {
                      0, ctx->stream, a, counts);
   QF_CHECK_HIP(hipGetLastError());
   scan_exclusive_i64(ctx, counts, counts, m, scratch);
-  if (shared && a.nkeys > 1)
+  // The multi-key kernel's LDS (staged tile plus every key's counts and bases) fits a CU up to
+  // seven columns; wider tables take the per-key kernel, which reads the shared input once per key.
+  if (shared && a.nkeys > 1 && a.ncols <= 7)
     launch_scatter<true>(ctx, a, counts);
   else
     launch_scatter<false>(ctx, a, counts);
