@@ -331,7 +331,7 @@ __device__ inline void load_tile(const PassArgs& a, int ki, int64_t tile, int la
 
 // Persistent workgroups, software pipelined: a tile's rows are loaded into registers while the
 // previous tile is written out of LDS.  A workgroup walks the tiles of all keys' inputs, key after
-// key (a first pass over one input for several keys is k_part_scatter_keys).
+// key (one key, or tables too wide for k_part_scatter_keys, which takes several keys together).
 // NC = a.ncols.  4096-row tiles: at 7-bit digits a bin's run in a tile averages 32 rows (128 B per
 // column), which measured 10-16 % faster than 2048-row tiles at the same number of waves per CU.
 template <int NC>
@@ -459,11 +459,28 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
-// The first pass over one input for every key (the binding table by each clause join variable):
-// the tile's digit counts of all keys are taken in one phase and scanned in one phase (a thread
-// per key and bin), then the tile is staged and written out once per key: two barriers per key
-// after the first instead of five.
-template <int NC>
+// Every key's key column of one tile (rows past the end read as 0).
+__device__ inline void load_key_columns(const PassArgs& a, int64_t tile, int lane, int w,
+                                        int32_t (&kv)[PT_MAXKEYS][PT_IPT]) {
+  const int64_t wave_base = tile * PT_TILE + (int64_t)w * 64 * PT_IPT;
+  const bool full = wave_base + 64 * PT_IPT <= a.n;
+#pragma unroll
+  for (int k = 0; k < PT_MAXKEYS; ++k) {
+    if (k >= a.nkeys) break;
+    const int32_t* src = a.in[k][a.key_col[k]] + wave_base;
+#pragma unroll
+    for (int i = 0; i < PT_IPT; ++i) kv[k][i] = full || wave_base + i * 64 + lane < a.n ? src[i * 64 + lane] : 0;
+  }
+}
+
+// Every key's tile of one tile index together (the binding table by each clause join variable):
+// the digit counts of all keys are taken in one phase and scanned in one phase (a thread per key
+// and bin), then each key's tile is staged and written out in turn: two barriers per key after
+// the first instead of five.  SHARED (a first pass over one input for every key): the tile is
+// read once for all keys.  Otherwise the key columns of all keys' tiles (their digits) load while
+// the tile before is written out, the first key's rows during the count and scan phases, and each
+// further key's rows while the key before is written out.
+template <int NC, bool SHARED>
 __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_part_scatter_keys(
     PassArgs a, const int64_t* bases) {
   extern __shared__ int32_t stage[];  // ncols x PT_TILE rows in bin order, then digits
@@ -486,7 +503,13 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
   TileWalk walk(T);
   for (int b = threadIdx.x; b < nk * (PT_WAVES / 2) * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0][0])[b] = 0;
   int32_t v[PT_IPT][NC];
-  if (walk.next < walk.end) load_tile<NC>(a, 0, walk.next, lane, w, v);
+  int32_t kv[SHARED ? 1 : PT_MAXKEYS][PT_IPT];  // !SHARED: every key's key column of the tile
+  if (walk.next < walk.end) {
+    if constexpr (SHARED)
+      load_tile<NC>(a, 0, walk.next, lane, w, v);
+    else
+      load_key_columns(a, walk.next, lane, w, kv);
+  }
 
   for (int64_t tile = walk.next; tile < walk.end; tile += walk.step) {
     const int64_t tile_base = tile * PT_TILE;
@@ -499,14 +522,21 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
       for (int k = 0; k < PT_MAXKEYS; ++k) {
         if (k >= nk) break;
-        const int kc = a.key_col[k];
-        int32_t key = v[i][0];
+        int32_t key;
+        if constexpr (SHARED) {
+          const int kc = a.key_col[k];
+          key = v[i][0];
 #pragma unroll
-        for (int c = 1; c < NC; ++c)
-          if (c == kc) key = v[i][c];
+          for (int c = 1; c < NC; ++c)
+            if (c == kc) key = v[i][c];
+        } else {
+          key = kv[k][i];
+        }
         d4[i] |= digit_of(key, a.mixed, a.shift, a.bits) << (8 * k);
       }
     }
+    // The first key's rows load during the count and scan phases.
+    if constexpr (!SHARED) load_tile<NC>(a, 0, tile, lane, w, v);
     __syncthreads();  // counts cleared; the previous tile's write-out is done with LDS
 #pragma unroll
     for (int i = 0; i < PT_IPT; ++i)
@@ -579,7 +609,14 @@ __global__ __launch_bounds__(PT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
         // No rank reads the counts any more: clear them for the next tile, whose loads fly while
         // this one is written out.
         for (int b = threadIdx.x; b < nk * (PT_WAVES / 2) * PT_MAXBINS; b += PT_THREADS) (&wrun[0][0][0])[b] = 0;
-        if (tile + walk.step < walk.end) load_tile<NC>(a, 0, tile + walk.step, lane, w, v);
+        if (tile + walk.step < walk.end) {
+          if constexpr (SHARED)
+            load_tile<NC>(a, 0, tile + walk.step, lane, w, v);
+          else
+            load_key_columns(a, tile + walk.step, lane, w, kv);
+        }
+      } else if (!SHARED) {
+        load_tile<NC>(a, k + 1, tile, lane, w, v);  // the staged rows no longer need the registers
       }
 
       // Consecutive threads store consecutive rows of a bin.
@@ -631,24 +668,24 @@ __global__ void k_part_offsets(OffsetArgs o, int stride, int64_t n, int mixed, i
   }
 }
 
-template <int NC>
+template <int NC, bool SHARED>
 static void launch_scatter_keys(Context* ctx, const PassArgs& a, const int64_t* bases) {
   const size_t lds = scatter_lds_bytes(NC);
   static bool lds_attr[64] = {};
   int dev = 0;
   QF_CHECK_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64 || !lds_attr[dev]) {
-    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter_keys<NC>),
+    QF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_part_scatter_keys<NC, SHARED>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (dev >= 0 && dev < 64) lds_attr[dev] = true;
   }
   int per_cu = 0;
-  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter_keys<NC>, PT_THREADS, lds));
+  QF_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_part_scatter_keys<NC, SHARED>, PT_THREADS, lds));
   QF_REQUIRE(per_cu >= 1, "the multi-key scatter does not fit a CU");
   int64_t grid = (int64_t)per_cu * ctx->num_cus;
   if (grid >= 8) grid &= ~int64_t(7);
   grid = std::min<int64_t>(grid, a.num_tiles);
-  hipLaunchKernelGGL((k_part_scatter_keys<NC>), dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
+  hipLaunchKernelGGL((k_part_scatter_keys<NC, SHARED>), dim3((unsigned)grid), dim3(PT_THREADS), lds, ctx->stream, a, bases);
 }
 
 template <int NC>
@@ -674,8 +711,10 @@ static void launch_scatter_one(Context* ctx, const PassArgs& a, const int64_t* b
 
 template <int NC, bool SHARED>
 static void launch_scatter_nc(Context* ctx, const PassArgs& a, const int64_t* bases) {
-  if constexpr (SHARED)
-    launch_scatter_keys<NC>(ctx, a, bases);
+  // The multi-key kernel's LDS (staged tile plus every key's counts and bases) fits a CU up to
+  // seven columns; wider tables take the per-key kernel, which reads a shared input once per key.
+  if (a.nkeys > 1 && NC <= 7)
+    launch_scatter_keys<NC, SHARED>(ctx, a, bases);
   else
     launch_scatter_one<NC>(ctx, a, bases);
 }
@@ -713,9 +752,7 @@ static void run_pass(Context* ctx, PassArgs a, bool shared, Scratch* scratch) {
                      0, ctx->stream, a, counts);
   QF_CHECK_HIP(hipGetLastError());
   scan_exclusive_i64(ctx, counts, counts, m, scratch);
-  // The multi-key kernel's LDS (staged tile plus every key's counts and bases) fits a CU up to
-  // seven columns; wider tables take the per-key kernel, which reads the shared input once per key.
-  if (shared && a.nkeys > 1 && a.ncols <= 7)
+  if (shared)
     launch_scatter<true>(ctx, a, counts);
   else
     launch_scatter<false>(ctx, a, counts);
