@@ -130,20 +130,25 @@ constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
 __device__ inline int64_t block_exclusive_scan_i64(int64_t v, int64_t* lds, int64_t* total) {
-  // Hillis-Steele over one value per thread (256 threads).
-  const int t = threadIdx.x;
-  lds[t] = v;
-  __syncthreads();
-  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-    int64_t add = t >= off ? lds[t - off] : 0;
-    __syncthreads();
-    lds[t] += add;
-    __syncthreads();
+  // Inclusive scan inside each wave by shuffles, then the four waves' totals through LDS.
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  long long incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const long long y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
   }
-  const int64_t incl = lds[t];
-  *total = lds[SCAN_THREADS - 1];
+  if (lane == 63) lds[w] = incl;
   __syncthreads();
-  return incl - v;
+  int64_t before = 0, all = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_THREADS / 64; ++i) {
+    const int64_t t = lds[i];
+    if (i < w) before += t;
+    all += t;
+  }
+  *total = all;
+  __syncthreads();  // lds is free again for the caller
+  return before + (int64_t)incl - v;
 }
 
 __global__ void k_scan_reduce(const int64_t* in, int64_t n, int64_t* block_sums) {
