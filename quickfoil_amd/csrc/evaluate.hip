@@ -36,6 +36,7 @@
 #include "qf_internal.h"
 
 #include <algorithm>
+#include <list>
 #include <map>
 #include <type_traits>
 #include <set>
@@ -1839,10 +1840,21 @@ struct RelPart {
   std::map<std::vector<int>, std::shared_ptr<TupleList>> tuples;
 };
 
+// A streamed launch's cell-by-list matrix and per-cell parts (k_ts_cells): they depend on the
+// launch's key-tuple lists only.
+struct CellMat {
+  DevBuf parts, cellmat;
+};
+
 struct RelCache {
   // Key: rows, join column, radix bits, then every column address.
   std::map<std::vector<int64_t>, std::shared_ptr<RelPart>> parts;
+  // Cell matrices of recent streamed launches, most recent first, keyed by cells, run lists and
+  // every list's per-cell offsets array (arrays of the cached key-tuple lists, which live as long
+  // as this cache).
+  std::list<std::pair<std::vector<int64_t>, std::shared_ptr<CellMat>>> cellmats;
 };
+constexpr size_t EV_CELLMAT_CACHE = 4;
 
 static RelCache& rel_cache(Context* ctx) {
   if (!ctx->eval_cache) ctx->eval_cache = std::make_shared<RelCache>();
@@ -2168,7 +2180,7 @@ struct StepPlan {
   std::vector<std::shared_ptr<TupleList>> keep_tuples;
   struct StreamLaunch {
     DevBuf d_launch, d_lists, d_targets;
-    DevBuf d_parts, d_cellmat;  // per cell: parts; per cell and list: (first entry, entries)
+    std::shared_ptr<CellMat> cm;  // per cell: parts; per cell and list: (first entry, entries)
     int64_t items_bound = 0;
   };
   std::vector<StreamLaunch> slaunches;      // units of the streamed form, in launches of bounded lists
@@ -2664,13 +2676,31 @@ static void plan_units(Context* ctx, const qf_table& binding, const qf_table* re
       tl.cells = cells;
       SL.d_launch = DevBuf(ctx, sizeof(TsLaunch));
       upload(ctx, SL.d_launch.get(), &tl, sizeof(TsLaunch));
-      // The cells' list ranges depend on the relations' key-tuple lists only: made once here.
-      SL.d_parts = DevBuf(ctx, cells * 4);
-      SL.d_cellmat = DevBuf(ctx, (size_t)cells * std::max<size_t>(1, pk.lists.size()) * sizeof(int2));
-      uploads_before(ctx);
-      hipLaunchKernelGGL(k_ts_cells, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, ctx->stream,
-                         SL.d_launch.as<TsLaunch>(), SL.d_parts.as<int32_t>(), SL.d_cellmat.as<int2>());
-      QF_CHECK_HIP(hipGetLastError());
+      // The cells' list ranges depend on the relations' key-tuple lists only: with cached
+      // relations a matrix made for the same lists before is taken again.
+      std::vector<int64_t> cm_key{cells, pk.nrun};
+      for (const TsList& l : pk.lists) cm_key.push_back((int64_t)(uintptr_t)l.off);
+      RelCache& rc = rel_cache(ctx);
+      if (cache)
+        for (auto it = rc.cellmats.begin(); it != rc.cellmats.end(); ++it)
+          if (it->first == cm_key) {
+            SL.cm = it->second;
+            rc.cellmats.splice(rc.cellmats.begin(), rc.cellmats, it);
+            break;
+          }
+      if (!SL.cm) {
+        SL.cm = std::make_shared<CellMat>();
+        SL.cm->parts = DevBuf(ctx, cells * 4);
+        SL.cm->cellmat = DevBuf(ctx, (size_t)cells * std::max<size_t>(1, pk.lists.size()) * sizeof(int2));
+        uploads_before(ctx);
+        hipLaunchKernelGGL(k_ts_cells, dim3((unsigned)ceil_div(cells, 256)), dim3(256), 0, ctx->stream,
+                           SL.d_launch.as<TsLaunch>(), SL.cm->parts.as<int32_t>(), SL.cm->cellmat.as<int2>());
+        QF_CHECK_HIP(hipGetLastError());
+        if (cache) {
+          rc.cellmats.emplace_front(std::move(cm_key), SL.cm);
+          if (rc.cellmats.size() > EV_CELLMAT_CACHE) rc.cellmats.pop_back();
+        }
+      }
       // Items: at most TS_CMAX binding chunks per cell times its parts, which number at most
       // one per cell with entries plus the cell's wave chunks over TS_PART (a cell's chunks are
       // at most its entries / TS_RUNROW, plus one row per run list, over TS_RUNROWS, plus two).
@@ -2821,18 +2851,18 @@ static void enqueue_step(Context* ctx, const qf_table& binding, int64_t num_posi
       {
         PhaseScope ps(ctx, "work_items");
         hipLaunchKernelGGL(k_ts_count, dim3((unsigned)ceil_div(cells_T, 256)), dim3(256), 0, ctx->stream,
-                           SL.d_launch.as<TsLaunch>(), P.d_light.as<uint8_t>(), SL.d_parts.as<int32_t>(),
+                           SL.d_launch.as<TsLaunch>(), P.d_light.as<uint8_t>(), SL.cm->parts.as<int32_t>(),
                            P.d_ts_counts.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
         scan_exclusive_i64(ctx, P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), cells_T);
         hipLaunchKernelGGL(k_ts_expand, dim3((unsigned)ceil_div(SL.items_bound, 256)), dim3(256), 0, ctx->stream,
-                           P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), SL.d_parts.as<int32_t>(), cells_T,
+                           P.d_ts_counts.as<int64_t>(), P.d_ts_starts.as<int64_t>(), SL.cm->parts.as<int32_t>(), cells_T,
                            P.d_ts_items.as<DevItem>(), P.d_ts_num.as<int64_t>());
         QF_CHECK_HIP(hipGetLastError());
       }
       TsArgs ta{};
       ta.launch = SL.d_launch.as<TsLaunch>();
-      ta.cellmat = SL.d_cellmat.as<int2>();
+      ta.cellmat = SL.cm->cellmat.as<int2>();
       ta.units = P.d_units.as<DevUnit>();
       ta.lits = P.d_lits.as<DevLit>();
       ta.items = P.d_ts_items.as<DevItem>();
